@@ -1,0 +1,222 @@
+/*
+ * jrhip.h -- C ABI of the MI355X rasterisation library (libjrhip.so).
+ *
+ * The library replaces the body of jaxrenderer's `pipeline.render`
+ * (renderer/pipeline.py:470-537) and the two shadow passes
+ * (renderer/shadow.py:40-125, renderer/renderer.py:346-382) with HIP kernels
+ * for gfx950.  Every pointer in these structs is a device pointer owned by the
+ * caller; the library never allocates device memory: scratch space comes in as
+ * a workspace argument sized by jr_workspace_size().  Every entry point is
+ * stream ordered on the hipStream_t it receives and returns 0 on success or a
+ * negative JR_E_* code; jr_last_error() returns the thread's last message.
+ *
+ * Buffer layout follows the reference's types (renderer/types.py:63-64,88):
+ * image buffers are x-major, [scene][width][height](,channel), float32, and a
+ * pixel (i, j) samples NDC ((i - vp[0][3]) / vp[0][0], (j - vp[1][3]) / vp[1][1])
+ * with integer i, j (renderer/pipeline.py:177).
+ *
+ * The same structs are consumed by the CPU oracle (oracle/jr_oracle.c), which
+ * takes host pointers instead.  This header is plain C.
+ */
+#ifndef JRHIP_H
+#define JRHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JR_ABI_VERSION 1
+
+/* Built-in shaders (renderer/shaders/ directory).  JR_SHADER_VISIBILITY runs only
+ * the default primitive chooser (renderer/shader.py:155-251) and the default
+ * fragment (renderer/shader.py:292-333): it writes depth, primitive id and
+ * barycentrics; the Python layer runs user-defined hooks on top of it. */
+enum {
+  JR_SHADER_DEPTH = 0,                   /* shaders/depth.py:38-61 */
+  JR_SHADER_GOURAUD = 1,                 /* shaders/gouraud.py:49-124 */
+  JR_SHADER_GOURAUD_TEXTURE = 2,         /* shaders/gouraud_texture.py:55-167 */
+  JR_SHADER_PHONG = 3,                   /* shaders/phong.py:65-187 */
+  JR_SHADER_PHONG_DARBOUX = 4,           /* shaders/phong_darboux.py:106-316 */
+  JR_SHADER_PHONG_REFLECTION = 5,        /* shaders/phong_reflection.py:91-258 */
+  JR_SHADER_PHONG_REFLECTION_SHADOW = 6, /* shaders/phong_reflection_shadow.py:99-295 */
+  JR_SHADER_VISIBILITY = 7,
+  JR_SHADER_COUNT = 8
+};
+
+/* error codes */
+enum {
+  JR_OK = 0,
+  JR_E_INVALID = -1,   /* bad argument (shape, null pointer, unknown shader) */
+  JR_E_WORKSPACE = -2, /* workspace too small */
+  JR_E_HIP = -3,       /* a HIP runtime call failed */
+  JR_E_UNSUPPORTED = -4
+};
+
+/* Per-scene uniforms.  One struct per scene, in device memory for
+ * jr_forward / jr_backward and host memory for the oracle.  Matrices are
+ * row-major 4x4 float32, as the reference's `Camera` fields
+ * (renderer/geometry.py:205-278). */
+typedef struct jr_scene {
+  int64_t tex_base;        /* float offset of the diffuse map [tex_w][tex_h][tex_c] */
+  int64_t spec_base;       /* float offset of the specular map [tex_w][tex_h] */
+  int64_t nmap_base;       /* float offset of the normal map [nmap_w][nmap_h][3] */
+  float world_to_clip[16]; /* Camera.world_to_clip */
+  float viewport[16];      /* Camera.viewport */
+  float world_to_eye_norm[16]; /* Camera.world_to_eye_norm */
+  float shadow_world_to_clip[16]; /* Shadow.camera.world_to_clip */
+  float shadow_viewport[16];      /* Shadow.camera.viewport */
+  float light_direction[3]; /* LightSource.direction as handed to the shader */
+  float light_colour[3];    /* LightSource.colour */
+  float light_dir_eye[3];   /* normalised light direction in eye space */
+  float ambient[3];
+  float diffuse[3];
+  float specular[3];
+  float shadow_strength[3]; /* Shadow.strength */
+  float pad_f;
+  int32_t tri_begin;       /* this scene owns faces rows [tri_begin, tri_end) */
+  int32_t tri_end;
+  int32_t tex_w, tex_h, tex_c;
+  int32_t nmap_w, nmap_h;
+  int32_t texshape_base;   /* int32-pair offset of MergedModel.texture_shape */
+  int32_t texshape_count;  /* number of objects (rows of texture_shape) */
+  int32_t tex_offset;      /* MergedModel.offset */
+  int32_t spec_w, spec_h;  /* specular map dims (clamp bounds of its reads) */
+} jr_scene;
+
+/* Geometry and maps shared by all scenes of a call.  A vertex "slot" is one
+ * gl_VertexID of the reference's vertex stage (renderer/pipeline.py:516-518);
+ * the per-slot attribute is attr[attr_index[slot]] (identity when the index
+ * pointer is NULL).  This expresses Renderer.render's per-corner flattening
+ * (renderer/renderer.py:277-296) without materialising it.  All indices are
+ * global (already offset per scene). */
+typedef struct jr_inputs {
+  const float* positions;       /* [*,3] world space */
+  const int32_t* pos_index;     /* [n_slots] or NULL */
+  const float* normals;         /* [*,3] world space */
+  const int32_t* nrm_index;
+  const float* uvs;             /* [*,2] */
+  const int32_t* uv_index;
+  const int32_t* tex_index;     /* [*] texture-map index (MergedModel.texture_index) */
+  const int32_t* tex_index_index;
+  const float* colours;         /* [*,3] per-vertex colour (gouraud) */
+  const int32_t* col_index;
+  const int32_t* faces;         /* [n_tris,3] slot ids: pipeline `face_indices` */
+  const int32_t* id_to_face;    /* darboux: [n_slots] face row per slot */
+  const int32_t* face_slots;    /* darboux: [*,3] slot ids per face row */
+  const float* textures;        /* flat diffuse maps */
+  const float* specular_maps;   /* flat specular maps */
+  const float* normal_maps;     /* flat normal maps (darboux) */
+  const int32_t* texture_shapes;/* flat [*,2] (phong_reflection*) */
+  const float* shadow_maps;     /* [n_scenes][shadow_w][shadow_h] */
+  const float* clip;            /* optional [n_slots,4] gl_Position computed by a
+                                   caller-side vertex stage; replaces
+                                   world_to_clip @ positions when set */
+  int32_t n_slots;
+  int32_t n_tris;
+  int32_t shadow_w;
+  int32_t shadow_h;
+} jr_inputs;
+
+/* Output buffers, [n_scenes][width][height](,3).  Pixels whose final `keep`
+ * is false are left untouched unless jr_frame.init_buffers is set
+ * (renderer/pipeline.py:401-440: keep ? new : old). */
+typedef struct jr_outputs {
+  float* zbuffer;   /* required */
+  float* canvas;    /* NULL for DEPTH / VISIBILITY */
+  int32_t* prim_id; /* optional: chosen face row where written, else -1 */
+  float* bary;      /* optional: perspective-correct barycentrics (3) */
+} jr_outputs;
+
+typedef struct jr_frame {
+  int64_t bin_capacity;   /* triangle-tile pairs the workspace can hold */
+  int32_t width;          /* buffer axis 0 */
+  int32_t height;         /* buffer axis 1 */
+  int32_t n_scenes;
+  int32_t shader;         /* JR_SHADER_* */
+  int32_t init_buffers;   /* 1: unwritten pixels take the defaults below */
+  int32_t apply_offset;   /* 1: zbuffer += depth_offset everywhere (shadow map) */
+  float zbuffer_default;
+  float colour_default[3];
+  float depth_offset;
+  int32_t tile_w;         /* pixels per tile along width (x) */
+  int32_t tile_h;         /* pixels per tile along height (y) */
+  int32_t persistent;     /* 1: one persistent raster launch draining a work queue */
+  int32_t pad_i;
+} jr_frame;
+
+/* Gradient buffers for jr_backward (all optional, accumulated with +=). */
+typedef struct jr_grads {
+  float* d_positions;   /* [*,3] scatter-added through pos_index */
+  float* d_colours;     /* [*,3] through col_index (gouraud) */
+  float* d_normals;     /* [*,3] through nrm_index */
+  float* d_textures;    /* flat, like jr_inputs.textures */
+  float* d_specular_maps;
+  float* d_scene;       /* [n_scenes][JR_DSCENE_FLOATS]: see JR_DSCENE_* */
+} jr_grads;
+
+/* Layout of one row of jr_grads.d_scene. */
+enum {
+  JR_DSCENE_LIGHT_DIRECTION = 0,  /* 3 */
+  JR_DSCENE_LIGHT_COLOUR = 3,     /* 3 */
+  JR_DSCENE_LIGHT_DIR_EYE = 6,    /* 3 */
+  JR_DSCENE_AMBIENT = 9,          /* 3 */
+  JR_DSCENE_DIFFUSE = 12,         /* 3 */
+  JR_DSCENE_SPECULAR = 15,        /* 3 */
+  JR_DSCENE_WORLD_TO_CLIP = 18,   /* 16 */
+  JR_DSCENE_VIEWPORT = 34,        /* 16 */
+  JR_DSCENE_WORLD_TO_EYE_NORM = 50, /* 16 */
+  JR_DSCENE_FLOATS = 66
+};
+
+typedef struct hipStream_t_opaque* jr_stream_t; /* a hipStream_t */
+
+/* ABI version of the loaded library (== JR_ABI_VERSION). */
+int32_t jr_version(void);
+
+/* Message of the calling thread's last failed call. */
+const char* jr_last_error(void);
+
+/* sizeof of the ABI structs, for binding-side layout checks:
+ * 0 jr_scene, 1 jr_inputs, 2 jr_outputs, 3 jr_frame, 4 jr_grads. */
+int64_t jr_sizeof(int32_t which);
+
+/* Bytes of workspace jr_forward needs for this frame (bin_capacity included).
+ * Replaces nothing in the reference: JAX sizes its temporaries itself. */
+int64_t jr_workspace_size(const jr_frame* frame, int32_t n_tris);
+
+/* Render n_scenes scenes in one call: vertex transform, triangle setup,
+ * screen-tile binning, per-tile visibility (default primitive chooser), the
+ * shader's interpolate/fragment/mix and the buffer merge.
+ * Replaces renderer/pipeline.py:470-537 `render(camera, shader, buffers,
+ * face_indices, extra, loop_unroll)` for the built-in shaders, and with
+ * frame->shader == JR_SHADER_DEPTH, apply_offset == 1 the shadow-map pass of
+ * renderer/shadow.py:40-125.  scenes is a device array of n_scenes structs. */
+int32_t jr_forward(const jr_frame* frame, const jr_inputs* inputs,
+                   const jr_scene* scenes, jr_outputs* outputs, void* workspace,
+                   int64_t workspace_bytes, jr_stream_t stream);
+
+/* Number of triangle-tile pairs the last jr_forward on `workspace` binned
+ * (device value, read asynchronously into *pairs on `stream`; the caller
+ * synchronises).  Used to size bin_capacity. */
+int32_t jr_binned_pairs(const void* workspace, const jr_frame* frame,
+                        int32_t n_tris, int64_t* pairs_host,
+                        jr_stream_t stream);
+
+/* Reverse-mode pass of jr_forward with visibility held fixed: takes the
+ * upstream gradients of canvas (and optionally zbuffer) plus the prim_id and
+ * bary buffers jr_forward wrote, recomputes fragment attributes from them and
+ * accumulates gradients into `grads`.  Replaces jax.grad through
+ * renderer/pipeline.py:283-328 and the shaders' fragment/vertex stages. */
+int32_t jr_backward(const jr_frame* frame, const jr_inputs* inputs,
+                    const jr_scene* scenes, const float* d_canvas,
+                    const float* d_zbuffer, const int32_t* prim_id,
+                    const float* bary, jr_grads* grads, jr_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JRHIP_H */

@@ -1,0 +1,205 @@
+// jr_math.h -- scalar float32 math of the rasterisation path, in the order the
+// reference evaluates it.  Device-only (included by the .hip sources); the CPU
+// oracle restates the same formulas separately in oracle/jr_oracle.c.
+//
+// Everything here must be compiled with -ffp-contract=off and without
+// fast-math: the primitive-id and depth buffers are compared bit-for-bit
+// against the oracle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define JR_DEV __device__ __forceinline__
+
+namespace jr {
+
+// ---- canonical small linear algebra -------------------------------------
+
+// Camera.apply (renderer/geometry.py:280-315): out_i = sum_j p_j * M[i][j],
+// summed left to right.
+JR_DEV void apply4(const float* __restrict__ M, float p0, float p1, float p2,
+                   float p3, float out[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float s = p0 * M[4 * i + 0];
+    s = s + p1 * M[4 * i + 1];
+    s = s + p2 * M[4 * i + 2];
+    s = s + p3 * M[4 * i + 3];
+    out[i] = s;
+  }
+}
+
+// jnp.dot of two 3-vectors, left to right.
+JR_DEV float dot3(const float a[3], const float b[3]) {
+  float s = a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+// geometry.normalise (renderer/geometry.py:36-47): v / ||v||_2 with
+// ||v|| = sqrt(sum v_i^2), each component divided.
+JR_DEV void normalise3(const float v[3], float out[3]) {
+  float ss = v[0] * v[0];
+  ss = ss + v[1] * v[1];
+  ss = ss + v[2] * v[2];
+  const float n = sqrtf(ss);
+  out[0] = v[0] / n;
+  out[1] = v[1] / n;
+  out[2] = v[2] / n;
+}
+
+// Camera.apply_vec (renderer/geometry.py:346-378): normalise, w = 0, apply,
+// drop w, normalise.
+JR_DEV void apply_vec(const float* __restrict__ M, const float v[3],
+                      float out[3]) {
+  float n[3];
+  normalise3(v, n);
+  float h[4];
+  apply4(M, n[0], n[1], n[2], 0.0f, h);
+  const float t[3] = {h[0], h[1], h[2]};
+  normalise3(t, out);
+}
+
+// jnp.linalg.det for 3x3: JAX's closed form _det_3x3 evaluated left to right
+// (renderer/pipeline.py:93).
+JR_DEV float det3(const float a[9]) {
+  float d = (a[0] * a[4]) * a[8];
+  d = d + (a[1] * a[5]) * a[6];
+  d = d + (a[2] * a[3]) * a[7];
+  d = d - (a[2] * a[4]) * a[6];
+  d = d - (a[0] * a[5]) * a[7];
+  d = d - (a[1] * a[3]) * a[8];
+  return d;
+}
+
+// jnp.linalg.inv (renderer/pipeline.py:105): LU with partial pivoting in the
+// operation order of LAPACK sgetrf2 (first maximal |pivot|, reciprocal scaling
+// when |pivot| >= FLT_MIN), then the permuted identity solved with the
+// reference-BLAS strsm loops (lower unit, then upper), skipping zero entries.
+JR_DEV void inv3(const float A[9], float X[9]) {
+  float a[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a[i] = A[i];
+  int perm[3] = {0, 1, 2};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    int p = j;
+    float amax = fabsf(a[3 * j + j]);
+#pragma unroll
+    for (int i = j + 1; i < 3; ++i) {
+      const float v = fabsf(a[3 * i + j]);
+      if (v > amax) {
+        amax = v;
+        p = i;
+      }
+    }
+    if (a[3 * p + j] != 0.0f) {
+      if (p != j) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float t = a[3 * p + k];
+          a[3 * p + k] = a[3 * j + k];
+          a[3 * j + k] = t;
+        }
+        const int tp = perm[p];
+        perm[p] = perm[j];
+        perm[j] = tp;
+      }
+      const float piv = a[3 * j + j];
+      if (fabsf(piv) >= 1.17549435e-38f) {
+        const float r = 1.0f / piv;
+#pragma unroll
+        for (int i = j + 1; i < 3; ++i) a[3 * i + j] = a[3 * i + j] * r;
+      } else {
+#pragma unroll
+        for (int i = j + 1; i < 3; ++i) a[3 * i + j] = a[3 * i + j] / piv;
+      }
+    }
+#pragma unroll
+    for (int i = j + 1; i < 3; ++i)
+#pragma unroll
+      for (int k = j + 1; k < 3; ++k)
+        a[3 * i + k] = a[3 * i + k] - a[3 * i + j] * a[3 * j + k];
+  }
+  float x[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[3 * r + c] = (perm[r] == c) ? 1.0f : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (x[3 * k + c] != 0.0f) {
+#pragma unroll
+        for (int i = k + 1; i < 3; ++i)
+          x[3 * i + c] = x[3 * i + c] - x[3 * k + c] * a[3 * i + k];
+      }
+    }
+#pragma unroll
+    for (int k = 2; k >= 0; --k) {
+      if (x[3 * k + c] != 0.0f) {
+        x[3 * k + c] = x[3 * k + c] / a[3 * k + k];
+#pragma unroll
+        for (int i = 0; i < k; ++i)
+          x[3 * i + c] = x[3 * i + c] - x[3 * k + c] * a[3 * i + k];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) X[i] = x[i];
+}
+
+// ---- scalar conversions with JAX semantics --------------------------------
+
+// float32 -> int32 as XLA converts: NaN -> 0, saturate out-of-range values.
+JR_DEV int32_t f2i_sat(float f) {
+  if (!(f == f)) return 0;
+  if (f >= 2147483648.0f) return 2147483647;
+  if (f <= -2147483648.0f) return (int32_t)(-2147483647 - 1);
+  return (int32_t)f;
+}
+
+// NumPy/JAX integer index: negative values count from the end, then the
+// gather clamps to [0, n-1] (JAX's default out-of-bounds mode for reads).
+JR_DEV int32_t index_clamp(int32_t i, int32_t n) {
+  if (i < 0) i += n;
+  if (i < 0) i = 0;
+  if (i > n - 1) i = n - 1;
+  return i;
+}
+
+// Python/jnp integer `%` with a positive divisor: result in [0, n).
+JR_DEV int32_t floor_mod(int32_t a, int32_t n) {
+  int32_t r = a % n;
+  if (r != 0 && ((r < 0) != (n < 0))) r += n;
+  return r;
+}
+
+// lax.max / jnp.maximum: NaN propagates.
+JR_DEV float max_nan(float a, float b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+
+// jnp.modf fractional part: x - (x >= 0 ? floor(x) : ceil(x)).
+JR_DEV float modf_frac(float x) {
+  const float whole = (x >= 0.0f) ? floorf(x) : ceilf(x);
+  return x - whole;
+}
+
+// Total order key of a depth for the argmin: NaN first (jnp.argmin returns the
+// first NaN), -0 == +0, otherwise the IEEE order.
+JR_DEV uint32_t depth_key(float z) {
+  if (z != z) return 0u;
+  if (z == 0.0f) z = 0.0f;
+  uint32_t u = __float_as_uint(z);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+constexpr uint32_t kInfKey = 0xFF800000u;  // depth_key(+inf)
+
+}  // namespace jr
