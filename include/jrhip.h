@@ -205,6 +205,15 @@ int32_t jr_binned_pairs(const void* workspace, const jr_frame* frame,
                         int32_t n_tris, int64_t* pairs_host,
                         jr_stream_t stream);
 
+/* Instrumentation (no counterpart in the reference, whose notebooks time whole
+ * jitted calls): when enabled, each jr_forward records HIP events on its own
+ * stream around its four stages (setup, scan, bin, raster).
+ * jr_stage_times waits for them and writes 4 floats (ms) per recorded call,
+ * oldest first, for up to max_calls calls; returns the number written and
+ * forgets all recorded calls. */
+int32_t jr_stage_timing(int32_t enable);
+int32_t jr_stage_times(float* ms, int32_t max_calls);
+
 /* Reverse-mode pass of jr_forward with visibility held fixed: takes the
  * upstream gradients of canvas (and optionally zbuffer) plus the prim_id and
  * bary buffers jr_forward wrote, recomputes fragment attributes from them and

@@ -165,6 +165,10 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.jr_forward.restype = c_i32
     lib.jr_binned_pairs.argtypes = [c_vp, P(JrFrame), c_i32, P(c_i64), c_vp]
     lib.jr_binned_pairs.restype = c_i32
+    lib.jr_stage_timing.argtypes = [c_i32]
+    lib.jr_stage_timing.restype = c_i32
+    lib.jr_stage_times.argtypes = [P(c_f), c_i32]
+    lib.jr_stage_times.restype = c_i32
     lib.jr_backward.argtypes = [
         P(JrFrame), P(JrInputs), c_vp, c_vp, c_vp, c_vp, c_vp, P(JrGrads), c_vp
     ]

@@ -1,0 +1,262 @@
+"""Batched rendering of many scenes of uneven size in one pass.
+
+The reference batches with jax.vmap over equally-shaped scenes
+(renderer/model.py:424-442, examples/batch_rendering.py:83-95).  Here scenes
+are packed CSR-style -- triangles, vertices and texture maps of every scene
+concatenated, with one jr_scene record per scene giving its triangle range,
+map offsets, camera and light -- so scenes of 50 and 8000 triangles share one
+launch of each kernel stage.
+
+`plan_batch` builds the calls (shadow-map pass through the light camera with a
+map of its own size, as renderer/shadow.py:40-125 does through
+Shadow.render_shadow_map, then the phong_reflection(_shadow) pass of
+renderer/renderer.py:277-385).  A batch of one scene is exactly the plain
+per-scene forward.  `BatchRenderer` keeps everything on the device, reuses
+its outputs and workspaces, and can capture the whole sequence in a HIP graph.
+"""
+
+from __future__ import annotations
+
+from typing import Dict, List, NamedTuple, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _hip, _plan, _raster
+from ._plan import RasterCall
+from .geometry import Camera, asf32, normalise
+from .model import MergedModel
+from .renderer import LightParameters
+from .shadow import Shadow
+
+f32 = np.float32
+
+
+class BatchScene(NamedTuple):
+    model: MergedModel
+    camera: Camera
+    light: LightParameters
+
+
+class ShadowMapSpec(NamedTuple):
+    """Shadow.render_shadow_map arguments shared by the batch."""
+
+    width: int = 256
+    height: int = 256
+    centre: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    up: Tuple[float, float, float] = (0.0, 0.0, 1.0)
+    strength: Tuple[float, float, float] = (0.6, 0.6, 0.6)
+    offset: float = 0.05
+    distance: float = 10.0
+
+
+def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
+               shadow: Optional[ShadowMapSpec] = None, *, zbuffer_default: float = 1.0,
+               colour_default=(1.0, 1.0, 1.0)) -> List[RasterCall]:
+    pos, nrm, uvs = [], [], []
+    pos_index, nrm_index, uv_index, slot_tex, shadow_faces = [], [], [], [], []
+    tex, spec, tshape = [], [], []
+    main_sc, shadow_sc = [], []
+    voff = noff = uoff = toff = tex_off = spec_off = ts_off = 0
+    if shadow is not None:
+        svp = Camera.viewport_matrix(np.zeros(2, np.int32),
+                                     np.array((shadow.width, shadow.height)), 1.0)
+    for s in scenes:
+        m = s.model
+        faces = np.asarray(m.faces, np.int32)
+        F = int(faces.shape[0])
+        fuv = np.asarray(m.faces_uv, np.int32).reshape(-1)
+        ti = np.asarray(m.texture_index, np.int32)
+        pos.append(asf32(m.verts))
+        nrm.append(asf32(m.norms))
+        uvs.append(asf32(m.uvs))
+        pos_index.append(faces.reshape(-1) + voff)
+        nrm_index.append(np.asarray(m.faces_norm, np.int32).reshape(-1) + noff)
+        uv_index.append(fuv + uoff)
+        slot_tex.append(ti[np.clip(fuv, 0, max(len(ti) - 1, 0))])
+        shadow_faces.append(faces + voff)
+        dm = asf32(m.diffuse_map)
+        sm = asf32(m.specular_map)
+        ts = np.asarray(m.texture_shape, np.int32)
+        tex.append(dm.reshape(-1))
+        spec.append(sm.reshape(-1))
+        tshape.append(ts)
+        light = s.light
+        light_dir = normalise(asf32(light.direction))
+        sc = dict(
+            world_to_clip=s.camera.world_to_clip,
+            viewport=s.camera.viewport,
+            world_to_eye_norm=s.camera.world_to_eye_norm,
+            light_direction=light_dir,
+            light_colour=asf32(light.colour),
+            light_dir_eye=Camera.apply_vec(light_dir, s.camera.view),
+            ambient=asf32(light.ambient),
+            diffuse=asf32(light.diffuse),
+            specular=asf32(light.specular),
+            tri_begin=toff,
+            tri_end=toff + F,
+            tex_base=tex_off,
+            tex_shape=(dm.shape[0], dm.shape[1], dm.shape[2]),
+            spec_base=spec_off,
+            spec_shape=sm.shape[:2],
+            texshape_base=ts_off,
+            texshape_count=int(ts.shape[0]),
+            tex_offset=int(m.offset),
+        )
+        if shadow is not None:
+            scam = Shadow.camera_for(light.direction, svp, shadow.centre, shadow.up,
+                                     shadow.distance)
+            sc.update(shadow_world_to_clip=scam.world_to_clip, shadow_viewport=scam.viewport,
+                      shadow_strength=asf32(shadow.strength))
+            shadow_sc.append(_raster.scene_struct(world_to_clip=scam.world_to_clip,
+                                                  viewport=scam.viewport, tri_begin=toff,
+                                                  tri_end=toff + F))
+        main_sc.append(_raster.scene_struct(**sc))
+        voff += len(m.verts)
+        noff += len(m.norms)
+        uoff += len(m.uvs)
+        toff += F
+        tex_off += dm.size
+        spec_off += sm.size
+        ts_off += int(ts.shape[0])
+    positions = np.concatenate(pos)
+    inputs = {
+        "positions": positions,
+        "pos_index": np.concatenate(pos_index),
+        "normals": np.concatenate(nrm),
+        "nrm_index": np.concatenate(nrm_index),
+        "uvs": np.concatenate(uvs),
+        "uv_index": np.concatenate(uv_index),
+        "tex_index": np.concatenate(slot_tex),
+        "faces": np.arange(3 * toff, dtype=np.int32).reshape(toff, 3),
+        "textures": np.concatenate(tex),
+        "specular_maps": np.concatenate(spec),
+        "texture_shapes": np.concatenate(tshape),
+    }
+    calls: List[RasterCall] = []
+    shader = _hip.SHADER_PHONG_REFLECTION
+    shadow_shape = (0, 0)
+    if shadow is not None:
+        calls.append(RasterCall(
+            shader=_hip.SHADER_DEPTH, width=shadow.width, height=shadow.height, scenes=shadow_sc,
+            inputs={"positions": positions, "faces": np.concatenate(shadow_faces)},
+            n_slots=len(positions), n_tris=toff, init_buffers=True,
+            zbuffer_default=float(np.finfo(f32).max), apply_offset=True,
+            depth_offset=float(shadow.offset), want_canvas=False, tag="shadow"))
+        shader = _hip.SHADER_PHONG_REFLECTION_SHADOW
+        shadow_shape = (shadow.width, shadow.height)
+    calls.append(RasterCall(
+        shader=shader, width=width, height=height, scenes=main_sc, inputs=inputs,
+        n_slots=3 * toff, n_tris=toff, shadow_shape=shadow_shape, init_buffers=True,
+        zbuffer_default=float(zbuffer_default),
+        colour_default=tuple(float(c) for c in np.asarray(colour_default, f32).reshape(-1)[:3]),
+        shadow_from="shadow" if shadow is not None else None, tag="main"))
+    return calls
+
+
+class _Pass:
+    def __init__(self, call: RasterCall, dev: torch.device, tile, persistent: bool,
+                 want_ids: bool):
+        self.call = call
+        self.tile = tile
+        self.persistent = persistent
+        self.inputs = {k: _plan.to_device(v, dev, k) for k, v in call.inputs.items()}
+        self.scenes = _raster.scenes_to_device(call.scenes, dev)
+        B, W, H = call.n_scenes, call.width, call.height
+        self.zbuffer = torch.empty((B, W, H), dtype=torch.float32, device=dev)
+        self.canvas = (torch.empty((B, W, H, 3), dtype=torch.float32, device=dev)
+                       if call.want_canvas else None)
+        self.prim_id = torch.empty((B, W, H), dtype=torch.int32, device=dev) if want_ids else None
+        self.bary = torch.empty((B, W, H, 3), dtype=torch.float32, device=dev) if want_ids else None
+        self.set_capacity(_raster.default_capacity(call.n_tris, self.n_tiles))
+
+    @property
+    def n_tiles(self) -> int:
+        c = self.call
+        return _raster.n_tiles(c.width, c.height, c.n_scenes, self.tile)
+
+    def set_capacity(self, cap: int) -> None:
+        c = self.call
+        self.capacity = int(cap)
+        need = _raster.workspace_size(width=c.width, height=c.height, n_scenes=c.n_scenes,
+                                      n_tris=c.n_tris, tile=self.tile, capacity=self.capacity)
+        self.workspace = torch.empty(need, dtype=torch.uint8, device=self.zbuffer.device)
+
+    def launch(self) -> None:
+        c = self.call
+        _raster.forward(
+            shader=c.shader, width=c.width, height=c.height, scenes=self.scenes,
+            n_scenes=c.n_scenes, inputs=self.inputs, n_slots=c.n_slots, n_tris=c.n_tris,
+            zbuffer=self.zbuffer, canvas=self.canvas, prim_id=self.prim_id, bary=self.bary,
+            shadow_shape=c.shadow_shape, init_buffers=c.init_buffers,
+            zbuffer_default=c.zbuffer_default, colour_default=c.colour_default,
+            apply_offset=c.apply_offset, depth_offset=c.depth_offset, tile=self.tile,
+            persistent=self.persistent, capacity=self.capacity, workspace=self.workspace)
+
+    def pairs(self) -> int:
+        c = self.call
+        return _raster.binned_pairs(width=c.width, height=c.height, n_scenes=c.n_scenes,
+                                    n_tris=c.n_tris, tile=self.tile, capacity=self.capacity,
+                                    workspace=self.workspace)
+
+
+class BatchRenderer:
+    """Device-resident batched renderer over a plan from `plan_batch`."""
+
+    def __init__(self, calls: List[RasterCall], *, tile=(8, 8), shadow_tile=None,
+                 persistent: bool = True, want_ids: bool = False):
+        dev = _raster.require_device()
+        self.passes: Dict[str, _Pass] = {}
+        for c in calls:
+            t = shadow_tile if (c.tag == "shadow" and shadow_tile is not None) else tile
+            self.passes[c.tag] = _Pass(c, dev, t, persistent, want_ids and c.tag == "main")
+        main = self.passes["main"]
+        if main.call.shadow_from is not None:
+            main.inputs["shadow_maps"] = self.passes[main.call.shadow_from].zbuffer
+        self.graph: Optional[torch.cuda.CUDAGraph] = None
+
+    def render(self) -> Dict[str, torch.Tensor]:
+        for p in self.passes.values():
+            p.launch()
+        main = self.passes["main"]
+        out = {"zbuffer": main.zbuffer, "canvas": main.canvas}
+        if main.prim_id is not None:
+            out.update(prim_id=main.prim_id, bary=main.bary)
+        if "shadow" in self.passes:
+            out["shadow_map"] = self.passes["shadow"].zbuffer
+        return out
+
+    def calibrate(self, slack: float = 1.25) -> Dict[str, int]:
+        """Runs once and sizes every pass's bin capacity to the binned pairs
+        (times `slack`), so no tile falls back to the brute-force path."""
+        self.render()
+        torch.cuda.synchronize()
+        got = {}
+        for tag, p in self.passes.items():
+            n = p.pairs()
+            got[tag] = n
+            p.set_capacity(int(n * slack) + 1024)
+        self.graph = None
+        return got
+
+    def capture(self) -> None:
+        """Captures render() into a HIP graph (torch.cuda.CUDAGraph)."""
+        self.render()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.render()
+        self.graph = g
+
+    def replay(self) -> Dict[str, torch.Tensor]:
+        if self.graph is None:
+            self.capture()
+        self.graph.replay()
+        main = self.passes["main"]
+        out = {"zbuffer": main.zbuffer, "canvas": main.canvas}
+        if main.prim_id is not None:
+            out.update(prim_id=main.prim_id, bary=main.bary)
+        return out
+
+
+__all__ = ["BatchRenderer", "BatchScene", "ShadowMapSpec", "plan_batch"]

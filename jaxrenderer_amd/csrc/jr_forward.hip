@@ -23,9 +23,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/jrhip.h"
 #include "jr_math.h"
@@ -513,6 +516,35 @@ __global__ __launch_bounds__(TW* TH) void k_raster_persistent(RasterArgs a,
 // ---------------------------------------------------------------------------
 thread_local std::string g_err;
 
+// Optional per-stage timing: when enabled, every jr_forward records five HIP
+// events on its own stream (start, after setup, after scan, after binning,
+// after raster); jr_stage_times reads them back.
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::vector<std::array<hipEvent_t, 5>> g_timed;
+
+struct StageTimer {
+  hipStream_t st;
+  std::array<hipEvent_t, 5> ev{};
+  bool on = false;
+  explicit StageTimer(hipStream_t s) : st(s) {
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    on = g_timing;
+    if (!on) return;
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    mark(0);
+  }
+  void mark(int i) {
+    if (on && ev[i]) (void)hipEventRecord(ev[i], st);
+  }
+  ~StageTimer() {
+    if (!on) return;
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    g_timed.push_back(ev);
+  }
+};
+
 int32_t fail(int32_t code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -690,6 +722,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   int32_t* block_sums = reinterpret_cast<int32_t*>(ws + L.block_sums);
   int32_t* bins = reinterpret_cast<int32_t*>(ws + L.bins);
 
+  StageTimer timer(st);
   // counts, cursor and misc are contiguous: clear them with one fill kernel
   // (a kernel rather than hipMemsetAsync so the sequence can be graph-captured)
   const int64_t nz = (L.offsets - L.counts) / 4;
@@ -700,16 +733,19 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
     hipLaunchKernelGGL(k_setup, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
                        scenes, f, rec, rects, counts);
   }
+  timer.mark(1);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
                      dim3(kScanThreads), 0, st, counts, offsets, block_sums, L.n_tiles);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, block_sums,
                      L.nblk, misc);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
                      dim3(kScanThreads), 0, st, offsets, block_sums, L.n_tiles);
+  timer.mark(2);
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
                        scenes, f, rec, rects, offsets, cursor, bins);
   }
+  timer.mark(3);
   RasterArgs a;
   a.in = *in;
   a.scenes = scenes;
@@ -729,12 +765,46 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.depth_offset = fr->depth_offset;
   if (n_tris == 0) {
     // no primitive: every pixel keeps its old value (plus the offset)
-    if (!fr->init_buffers && !fr->apply_offset) return JR_OK;
+    if (!fr->init_buffers && !fr->apply_offset) {
+      timer.mark(4);
+      return JR_OK;
+    }
   }
   launch_raster(fr->shader, a, L.n_tiles, fr->persistent != 0, st);
+  timer.mark(4);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(JR_E_HIP, std::string("launch failed: ") + hipGetErrorString(err));
   return JR_OK;
+}
+
+int32_t jr_stage_timing(int32_t enable) {
+  std::lock_guard<std::mutex> g(g_timing_mu);
+  g_timing = enable != 0;
+  return JR_OK;
+}
+
+int32_t jr_stage_times(float* ms, int32_t max_calls) {
+  std::vector<std::array<hipEvent_t, 5>> calls;
+  {
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    calls.swap(g_timed);
+  }
+  int32_t n = 0;
+  for (auto& ev : calls) {
+    if (n < max_calls && ms) {
+      (void)hipEventSynchronize(ev[4]);
+      for (int k = 0; k < 4; ++k) {
+        float t = -1.0f;
+        if (ev[k] && ev[k + 1] && hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess)
+          t = -1.0f;
+        ms[4 * n + k] = t;
+      }
+      ++n;
+    }
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  return n;
 }
 
 int32_t jr_binned_pairs(const void* workspace, const jr_frame* frame, int32_t n_tris,

@@ -1,0 +1,158 @@
+"""Seeded synthetic workloads used by the benchmark and the tests.
+
+* config 1: the cube of the reference's examples (renderer/shapes/cube.py),
+  phong_reflection, 320x240.
+* config 2: one UV sphere of 48 x 96 segments with per-vertex radial noise,
+  256x256 diffuse / specular maps, phong_reflection, 960x540.
+* config 3: Brax-like scenes -- a ground plane plus 4..30 boxes, capsules
+  (8 x 16 segments) and spheres (12 x 24 segments) at poses U[-3,3]^3 with
+  uniform random orientations and 8x8 textures -- rendered at 84x84 with
+  phong_reflection_shadow and a 256x256 shadow map.
+
+Everything is drawn from numpy.random.Generator(PCG64(seed)).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from .batch import BatchScene, ShadowMapSpec
+from .geometry import Camera, transform_matrix_from_rotation
+from .model import Model, ModelObject, merge_objects
+from .renderer import CameraParameters, LightParameters, Renderer
+from .shapes.cube import create_cube
+
+f32 = np.float32
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def uv_sphere(n_lat: int, n_lon: int, radius=1.0, half_height: float = 0.0,
+              radial: np.ndarray = None) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """(verts, normals, uvs, faces) of a latitude/longitude sphere.  Rings of
+    the upper hemisphere move up by half_height and the lower ones down (a
+    capsule).  Quads touching a pole yield one degenerate triangle each."""
+    lat = np.linspace(0.0, math.pi, n_lat + 1)
+    lon = np.linspace(0.0, 2.0 * math.pi, n_lon + 1)
+    th, ph = np.meshgrid(lat, lon, indexing="ij")
+    n = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=-1)
+    r = radius if radial is None else radial.reshape(n_lat + 1, n_lon + 1, 1)
+    v = n * r
+    v[..., 2] += np.where(n[..., 2] > 0, half_height, np.where(n[..., 2] < 0, -half_height, 0.0))
+    uv = np.stack([ph / (2 * math.pi), 1.0 - th / math.pi], axis=-1)
+    idx = np.arange((n_lat + 1) * (n_lon + 1)).reshape(n_lat + 1, n_lon + 1)
+    a, b = idx[:-1, :-1], idx[1:, :-1]
+    c, d = idx[1:, 1:], idx[:-1, 1:]
+    faces = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3),
+                            np.stack([a, c, d], -1).reshape(-1, 3)])
+    return (v.reshape(-1, 3).astype(f32), n.reshape(-1, 3).astype(f32),
+            uv.reshape(-1, 2).astype(f32), faces.astype(np.int32))
+
+
+def mesh_model(verts, normals, uvs, faces, diffuse, specular) -> Model:
+    return Model(verts=verts, norms=normals, uvs=uvs, faces=faces, faces_norm=faces,
+                 faces_uv=faces, diffuse_map=diffuse.astype(f32), specular_map=specular.astype(f32))
+
+
+def plane_model(half: float, diffuse, specular, uv_repeat: float = 4.0) -> Model:
+    v = np.array([[-half, -half, 0.0], [half, -half, 0.0], [half, half, 0.0], [-half, half, 0.0]], f32)
+    n = np.tile(np.array([0.0, 0.0, 1.0], f32), (4, 1))
+    uv = np.array([[0, 0], [uv_repeat, 0], [uv_repeat, uv_repeat], [0, uv_repeat]], f32)
+    faces = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return mesh_model(v, n, uv, faces, diffuse, specular)
+
+
+def random_quaternion(rng) -> np.ndarray:
+    """Uniform unit quaternion (w, x, y, z) (Shoemake's method)."""
+    u1, u2, u3 = rng.random(3)
+    q = np.array([math.sqrt(u1) * math.cos(2 * math.pi * u3),
+                  math.sqrt(1 - u1) * math.sin(2 * math.pi * u2),
+                  math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+                  math.sqrt(u1) * math.sin(2 * math.pi * u3)])
+    return q.astype(f32)
+
+
+NOTEBOOK_LIGHT = LightParameters(
+    direction=np.array((0.57735, -0.57735, 0.57735), f32),
+    colour=np.ones(3, f32),
+    ambient=np.full(3, 0.8, f32),
+    diffuse=np.full(3, 0.8, f32),
+    specular=np.full(3, 0.6, f32),
+)
+
+
+def brax_like_objects(rng) -> List[ModelObject]:
+    spec8 = np.full((8, 8), 2.0, f32)
+    objs = [ModelObject(model=plane_model(10.0, rng.random((8, 8, 3)).astype(f32), spec8),
+                        transform=np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, -3.5],
+                                            [0, 0, 0, 1]], f32))]
+    for _ in range(int(rng.integers(4, 31))):
+        kind = int(rng.integers(0, 3))
+        tex = rng.random((8, 8, 3)).astype(f32)
+        if kind == 0:
+            he = rng.uniform(0.2, 0.6, size=3).astype(f32)
+            model = create_cube(he, np.ones(2, f32), tex, spec8)
+        elif kind == 1:
+            r = float(rng.uniform(0.1, 0.3))
+            model = mesh_model(*uv_sphere(8, 16, r, float(rng.uniform(0.2, 0.5))), tex, spec8)
+        else:
+            model = mesh_model(*uv_sphere(12, 24, float(rng.uniform(0.2, 0.6))), tex, spec8)
+        t = np.identity(4, dtype=f32)
+        t[:3, :3] = transform_matrix_from_rotation(random_quaternion(rng))
+        t[:3, 3] = rng.uniform(-3.0, 3.0, size=3).astype(f32)
+        objs.append(ModelObject(model=model, transform=t))
+    return objs
+
+
+def ring_camera(i: int, n: int, width: int, height: int, radius: float = 8.0,
+                height_z: float = 4.0, fov: float = 60.0) -> Camera:
+    a = 2.0 * math.pi * i / max(n, 1)
+    cp = CameraParameters(viewWidth=width, viewHeight=height, hfov=fov, vfov=fov,
+                          position=(radius * math.cos(a), radius * math.sin(a), height_z),
+                          target=(0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0))
+    return Renderer.create_camera_from_parameters(cp)
+
+
+def config3(n_scenes: int = 1024, seed: int = 0, width: int = 84, height: int = 84,
+            shadow_size: int = 256) -> Tuple[List[BatchScene], ShadowMapSpec]:
+    rng = _rng(seed)
+    scenes = []
+    for i in range(n_scenes):
+        model = merge_objects(brax_like_objects(rng))
+        scenes.append(BatchScene(model=model, camera=ring_camera(i, n_scenes, width, height),
+                                 light=NOTEBOOK_LIGHT))
+    return scenes, ShadowMapSpec(width=shadow_size, height=shadow_size)
+
+
+def config2(seed: int = 0, width: int = 960, height: int = 540) -> BatchScene:
+    rng = _rng(seed)
+    n_lat, n_lon = 48, 96
+    noise = rng.normal(size=(n_lat + 1, n_lon + 1))
+    # smooth along both grid axes (wrapping in longitude)
+    for _ in range(4):
+        noise = (noise + np.roll(noise, 1, 1) + np.roll(noise, -1, 1)) / 3.0
+        noise[1:-1] = (noise[1:-1] + noise[:-2] + noise[2:]) / 3.0
+    radial = 1.0 + 0.05 * noise / max(noise.std(), 1e-6)
+    v, n, uv, f = uv_sphere(n_lat, n_lon, 1.0, 0.0, radial)
+    tex = rng.random((256, 256, 3)).astype(f32)
+    spec = rng.uniform(0.0, 4.0, size=(256, 256)).astype(f32)
+    model = merge_objects([ModelObject(model=mesh_model(v, n, uv, f, tex, spec))])
+    az = float(rng.uniform(0.0, 2.0 * math.pi))
+    hfov = float(np.degrees(2 * math.atan(math.tan(math.radians(30)) * width / height)))
+    cp = CameraParameters(viewWidth=width, viewHeight=height, vfov=60.0, hfov=hfov,
+                          position=(3.0 * math.cos(az), 3.0 * math.sin(az), 0.5),
+                          target=(0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0))
+    return BatchScene(model=model, camera=Renderer.create_camera_from_parameters(cp),
+                      light=LightParameters())
+
+
+def triangle_counts(scenes: Sequence[BatchScene]) -> np.ndarray:
+    return np.array([len(s.model.faces) for s in scenes])
+
+
+__all__ = ["config2", "config3", "brax_like_objects", "uv_sphere", "NOTEBOOK_LIGHT"]
