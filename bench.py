@@ -52,6 +52,7 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-scene", action="store_true")
+    ap.add_argument("--no-backward", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build renders on one GPU; --gpus must be 1 "
@@ -111,6 +112,28 @@ def main() -> None:
 
     # kept pixels of the main pass (for the algorithmic byte count)
     kept = int((zbuf_b < 1.0).sum())
+
+    # forward + backward: L2 image loss against a fixed synthetic target,
+    # gradients to vertices, normals, diffuse/specular maps and scene params
+    fwd_bwd_fps = None
+    grads_finite = None
+    if not args.no_backward:
+        br.enable_backward()
+        gen = torch.Generator(device="cuda").manual_seed(1)
+        target = torch.rand(canvas_b.shape, generator=gen, device="cuda")
+        br.capture_train(target)
+        for _ in range(args.warmup):
+            br.train_graph.replay()
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            br.train_graph.replay()
+        torch.cuda.synchronize()
+        t_fb = (time.perf_counter() - t_start) / args.steps
+        fwd_bwd_fps = B / t_fb
+        grads_finite = bool(all(torch.isfinite(g).all() for g in br.grads.values()))
+        log(f"fwd+bwd: {t_fb * 1e3:.3f} ms per step -> {fwd_bwd_fps:.0f} frames/s "
+            f"({fwd_bwd_fps / fps:.2f}x forward), gradients finite: {grads_finite}")
 
     # plain per-scene forward: the same pass sequence with a batch of one
     per_scene_fps = None
@@ -214,7 +237,9 @@ def main() -> None:
         "per_scene_fps": None if per_scene_fps is None else round(per_scene_fps, 1),
         "batched_over_per_scene": None if per_scene_fps is None else round(fps / per_scene_fps, 2),
         "batched_vs_per_scene_mismatched_scenes": mismatches,
-        "fwd_bwd_fps": None,
+        "fwd_bwd_fps": None if fwd_bwd_fps is None else round(fwd_bwd_fps, 1),
+        "fwd_bwd_over_fwd": None if fwd_bwd_fps is None else round(fwd_bwd_fps / fps, 3),
+        "fwd_bwd_grads_finite": grads_finite,
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "binned_pairs": pairs,
         "roofline": roofline,

@@ -168,7 +168,8 @@ enum {
   JR_DSCENE_WORLD_TO_CLIP = 18,   /* 16 */
   JR_DSCENE_VIEWPORT = 34,        /* 16 */
   JR_DSCENE_WORLD_TO_EYE_NORM = 50, /* 16 */
-  JR_DSCENE_FLOATS = 66
+  JR_DSCENE_SHADOW_STRENGTH = 66, /* 3 */
+  JR_DSCENE_FLOATS = 69
 };
 
 typedef struct hipStream_t_opaque* jr_stream_t; /* a hipStream_t */
@@ -214,15 +215,27 @@ int32_t jr_binned_pairs(const void* workspace, const jr_frame* frame,
 int32_t jr_stage_timing(int32_t enable);
 int32_t jr_stage_times(float* ms, int32_t max_calls);
 
-/* Reverse-mode pass of jr_forward with visibility held fixed: takes the
- * upstream gradients of canvas (and optionally zbuffer) plus the prim_id and
- * bary buffers jr_forward wrote, recomputes fragment attributes from them and
- * accumulates gradients into `grads`.  Replaces jax.grad through
- * renderer/pipeline.py:283-328 and the shaders' fragment/vertex stages. */
+/* Reverse-mode pass of jr_forward with visibility held fixed (the primitive
+ * chosen per pixel, comparisons, floor/round lookups carry no gradient, as
+ * under jax.grad).  Takes the upstream gradients of the canvas (and
+ * optionally the zbuffer) and the prim_id buffer jr_forward wrote; for every
+ * written pixel it recomputes the chosen primitive's edge values,
+ * barycentrics and fragment attributes exactly as the forward did and
+ * accumulates (+=) gradients into `grads`.  Replaces jax.grad through
+ * renderer/pipeline.py:163-328 (edge functions, depth, interpolation), the
+ * shaders' vertex and fragment stages and jnp.linalg.inv
+ * (renderer/pipeline.py:105).  Supported shaders: DEPTH, GOURAUD,
+ * GOURAUD_TEXTURE, PHONG, PHONG_REFLECTION, PHONG_REFLECTION_SHADOW.
+ * frame->width/height/n_scenes/shader/tile_w/tile_h must match the forward
+ * call; the workspace is sized by jr_backward_workspace_size. */
+int64_t jr_backward_workspace_size(const jr_frame* frame, int32_t n_tris);
+/* Message of the calling thread's last failed jr_backward. */
+const char* jr_backward_last_error(void);
 int32_t jr_backward(const jr_frame* frame, const jr_inputs* inputs,
                     const jr_scene* scenes, const float* d_canvas,
                     const float* d_zbuffer, const int32_t* prim_id,
-                    const float* bary, jr_grads* grads, jr_stream_t stream);
+                    jr_grads* grads, void* workspace, int64_t workspace_bytes,
+                    jr_stream_t stream);
 
 #ifdef __cplusplus
 }

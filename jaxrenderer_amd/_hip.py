@@ -133,7 +133,7 @@ class JrGrads(ctypes.Structure):
     ]
 
 
-DSCENE_FLOATS = 66
+DSCENE_FLOATS = 69
 DSCENE = {
     "light_direction": 0,
     "light_colour": 3,
@@ -144,6 +144,7 @@ DSCENE = {
     "world_to_clip": 18,
     "viewport": 34,
     "world_to_eye_norm": 50,
+    "shadow_strength": 66,
 }
 
 STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads)
@@ -170,9 +171,12 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.jr_stage_times.argtypes = [P(c_f), c_i32]
     lib.jr_stage_times.restype = c_i32
     lib.jr_backward.argtypes = [
-        P(JrFrame), P(JrInputs), c_vp, c_vp, c_vp, c_vp, c_vp, P(JrGrads), c_vp
+        P(JrFrame), P(JrInputs), c_vp, c_vp, c_vp, c_vp, P(JrGrads), c_vp, c_i64, c_vp
     ]
     lib.jr_backward.restype = c_i32
+    lib.jr_backward_workspace_size.argtypes = [P(JrFrame), c_i32]
+    lib.jr_backward_workspace_size.restype = c_i64
+    lib.jr_backward_last_error.restype = ctypes.c_char_p
 
 
 def check_layout(lib: ctypes.CDLL) -> None:
@@ -208,5 +212,6 @@ def lib() -> ctypes.CDLL:
 
 def check(rc: int, what: str) -> None:
     if rc != 0:
-        msg = lib().jr_last_error().decode(errors="replace")
+        getter = lib().jr_backward_last_error if what == "jr_backward" else lib().jr_last_error
+        msg = getter().decode(errors="replace")
         raise HipError(f"{what} failed with code {rc}: {msg}")

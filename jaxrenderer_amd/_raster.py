@@ -266,3 +266,63 @@ def workspace_size(*, width, height, n_scenes, n_tris, tile=DEFAULT_TILE, capaci
     if n < 0:
         _hip.check(-1, "jr_workspace_size")
     return int(n)
+
+
+_GRAD_FIELDS = ("d_positions", "d_colours", "d_normals", "d_textures", "d_specular_maps",
+                "d_scene")
+
+
+def backward(
+    *,
+    shader: int,
+    width: int,
+    height: int,
+    scenes: torch.Tensor,
+    n_scenes: int,
+    inputs: Dict[str, Optional[torch.Tensor]],
+    n_slots: int,
+    n_tris: int,
+    prim_id: torch.Tensor,
+    d_canvas: Optional[torch.Tensor],
+    d_zbuffer: Optional[torch.Tensor],
+    grads: Dict[str, Optional[torch.Tensor]],
+    shadow_shape: Tuple[int, int] = (0, 0),
+    stream: Optional[torch.cuda.Stream] = None,
+    workspace: Optional[torch.Tensor] = None,
+) -> None:
+    """One jr_backward call: accumulates into the float32 tensors of `grads`
+    (keys d_positions, d_colours, d_normals, d_textures, d_specular_maps,
+    d_scene)."""
+    L = _hip.lib()
+    dev = prim_id.device
+    stream = stream or torch.cuda.current_stream(dev)
+    fr = make_frame(width=width, height=height, n_scenes=n_scenes, shader=shader)
+    jin = _hip.JrInputs()
+    for name, dt in _INPUT_FIELDS:
+        setattr(jin, name, _ptr(inputs.get(name), name, dt, dev))
+    jin.n_slots, jin.n_tris = int(n_slots), int(n_tris)
+    jin.shadow_w, jin.shadow_h = int(shadow_shape[0]), int(shadow_shape[1])
+    jg = _hip.JrGrads()
+    for name in _GRAD_FIELDS:
+        setattr(jg, name, _ptr(grads.get(name), name, torch.float32, dev))
+    need = backward_workspace_size(width=width, height=height, n_scenes=n_scenes,
+                                   shader=shader, n_tris=n_tris)
+    ws = workspace if workspace is not None else workspace_for(stream, need, dev)
+    if ws.numel() < need:
+        raise ValueError(f"backward workspace holds {ws.numel()} bytes, {need} needed")
+    rc = L.jr_backward(
+        ctypes.byref(fr), ctypes.byref(jin), ctypes.c_void_p(scenes.data_ptr()),
+        ctypes.c_void_p(_ptr(d_canvas, "d_canvas", torch.float32, dev)),
+        ctypes.c_void_p(_ptr(d_zbuffer, "d_zbuffer", torch.float32, dev)),
+        ctypes.c_void_p(_ptr(prim_id, "prim_id", torch.int32, dev)), ctypes.byref(jg),
+        ctypes.c_void_p(ws.data_ptr()), ctypes.c_int64(ws.numel()),
+        ctypes.c_void_p(stream.cuda_stream))
+    _hip.check(rc, "jr_backward")
+
+
+def backward_workspace_size(*, width, height, n_scenes, shader, n_tris) -> int:
+    fr = make_frame(width=width, height=height, n_scenes=n_scenes, shader=shader)
+    n = _hip.lib().jr_backward_workspace_size(ctypes.byref(fr), int(n_tris))
+    if n < 0:
+        raise ValueError("jr_backward_workspace_size rejected the frame")
+    return int(n)

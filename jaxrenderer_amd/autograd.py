@@ -1,0 +1,202 @@
+"""Differentiable rendering: jr_forward / jr_backward behind
+torch.autograd.Function.
+
+The reference differentiates its pipeline with jax.grad
+(tests/smoke_test_grad.py:92-128).  Here `render_differentiable` runs the HIP
+forward, keeps only the per-pixel primitive ids, and its backward calls the
+hand-written jr_backward kernels, which recompute every fragment from those
+ids.  Gradients reach vertex positions, normals, per-vertex colours, texture
+and specular maps, and -- through one per-scene parameter vector laid out like
+include/jrhip.h's JR_DSCENE_* -- the light, the material coefficients, the
+shadow strength and the camera matrices (world_to_clip, viewport,
+world_to_eye_norm).  Visibility is piecewise constant, as under jax.grad.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _hip, _plan, _raster
+from ._plan import RasterCall
+from .pipeline import build_call, is_builtin
+
+_D = _hip.DSCENE
+_SLICES = {
+    "light_direction": (_D["light_direction"], 3),
+    "light_colour": (_D["light_colour"], 3),
+    "light_dir_eye": (_D["light_dir_eye"], 3),
+    "ambient": (_D["ambient"], 3),
+    "diffuse": (_D["diffuse"], 3),
+    "specular": (_D["specular"], 3),
+    "world_to_clip": (_D["world_to_clip"], 16),
+    "viewport": (_D["viewport"], 16),
+    "world_to_eye_norm": (_D["world_to_eye_norm"], 16),
+    "shadow_strength": (_D["shadow_strength"], 3),
+}
+
+
+def scene_rows(scenes: Sequence[_hip.JrScene]) -> np.ndarray:
+    """The differentiable fields of jr_scene records as [n_scenes, 69]."""
+    out = np.zeros((len(scenes), _hip.DSCENE_FLOATS), dtype=np.float32)
+    for i, s in enumerate(scenes):
+        for name, (o, n) in _SLICES.items():
+            out[i, o:o + n] = np.frombuffer(getattr(s, name), dtype=np.float32)[:n]
+    return out
+
+
+def scenes_with_rows(scenes: Sequence[_hip.JrScene], rows: np.ndarray) -> list:
+    """Copies of the jr_scene records with their differentiable fields
+    replaced by `rows`."""
+    out = []
+    for s, r in zip(scenes, rows):
+        c = _hip.JrScene.from_buffer_copy(bytes(s))
+        for name, (o, n) in _SLICES.items():
+            setattr(c, name, (ctypes.c_float * n)(*r[o:o + n]))
+        out.append(c)
+    return out
+
+
+_TENSOR_INPUTS = ("positions", "normals", "colours", "textures", "specular_maps")
+
+
+class RenderFunction(torch.autograd.Function):
+    """(params [B, 69], positions, normals, colours, textures, specular_maps)
+    -> (zbuffer [B, W, H], canvas [B, W, H, 3])."""
+
+    @staticmethod
+    def forward(ctx, call: RasterCall, params, positions, normals, colours, textures,
+                specular_maps):
+        dev = params.device
+        rows = params.detach().float().cpu().numpy()
+        scenes = _raster.scenes_to_device(scenes_with_rows(call.scenes, rows), dev)
+        inputs = {k: _plan.to_device(v, dev, k) for k, v in call.inputs.items()
+                  if k not in _TENSOR_INPUTS}
+        for name, t in zip(_TENSOR_INPUTS, (positions, normals, colours, textures, specular_maps)):
+            inputs[name] = None if t is None else t.detach().float().contiguous()
+        B, W, H = call.n_scenes, call.width, call.height
+        zbuffer = torch.empty((B, W, H), dtype=torch.float32, device=dev)
+        canvas = (torch.empty((B, W, H, 3), dtype=torch.float32, device=dev)
+                  if call.want_canvas else None)
+        prim_id = torch.empty((B, W, H), dtype=torch.int32, device=dev)
+        _raster.forward(
+            shader=call.shader, width=W, height=H, scenes=scenes, n_scenes=B, inputs=inputs,
+            n_slots=call.n_slots, n_tris=call.n_tris, zbuffer=zbuffer, canvas=canvas,
+            prim_id=prim_id, shadow_shape=call.shadow_shape, init_buffers=True,
+            zbuffer_default=call.zbuffer_default, colour_default=call.colour_default,
+            apply_offset=call.apply_offset, depth_offset=call.depth_offset)
+        ctx.call = call
+        ctx.scenes = scenes
+        ctx.inputs = inputs
+        ctx.prim_id = prim_id
+        ctx.shapes = [None if t is None else tuple(t.shape)
+                      for t in (positions, normals, colours, textures, specular_maps)]
+        ctx.needs = ctx.needs_input_grad
+        if canvas is None:
+            canvas = torch.zeros((B, W, H, 3), dtype=torch.float32, device=dev)
+        return zbuffer, canvas
+
+    @staticmethod
+    def backward(ctx, d_zbuffer, d_canvas):
+        call = ctx.call
+        dev = ctx.prim_id.device
+        grads: Dict[str, Optional[torch.Tensor]] = {}
+        names = ("d_positions", "d_normals", "d_colours", "d_textures", "d_specular_maps")
+        for i, (name, shape) in enumerate(zip(names, ctx.shapes)):
+            need = ctx.needs[2 + i] and shape is not None
+            grads[name] = torch.zeros(shape, dtype=torch.float32, device=dev) if need else None
+        grads["d_scene"] = (torch.zeros((call.n_scenes, _hip.DSCENE_FLOATS), dtype=torch.float32,
+                                        device=dev) if ctx.needs[1] else None)
+        dz = None if d_zbuffer is None else d_zbuffer.float().contiguous()
+        dc = None if (d_canvas is None or not call.want_canvas) else d_canvas.float().contiguous()
+        if dz is not None or dc is not None:
+            _raster.backward(
+                shader=call.shader, width=call.width, height=call.height, scenes=ctx.scenes,
+                n_scenes=call.n_scenes, inputs=ctx.inputs, n_slots=call.n_slots,
+                n_tris=call.n_tris, prim_id=ctx.prim_id, d_canvas=dc, d_zbuffer=dz,
+                grads=grads, shadow_shape=call.shadow_shape)
+        return (None, grads["d_scene"], grads["d_positions"], grads["d_normals"],
+                grads["d_colours"], grads["d_textures"], grads["d_specular_maps"])
+
+
+def _as_tensor(x, dev) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(dev)
+    return torch.as_tensor(np.asarray(x, dtype=np.float32), device=dev)
+
+
+def _row_tensor(dev, camera, extra, call: RasterCall) -> torch.Tensor:
+    """[1, 69] parameter row assembled (differentiably) from the camera and the
+    extra input's tensors, defaults from the call's scene record."""
+    base = torch.as_tensor(scene_rows(call.scenes)[0], device=dev)
+    parts = []
+    pos = 0
+    vals = {}
+    light = getattr(extra, "light", None)
+    if light is not None:
+        vals["light_direction"] = light.direction
+        vals["light_colour"] = light.colour
+    for name in ("light_dir_eye", "ambient", "diffuse", "specular"):
+        if hasattr(extra, name):
+            vals[name] = getattr(extra, name)
+    vals["world_to_clip"] = camera.world_to_clip
+    vals["viewport"] = camera.viewport
+    vals["world_to_eye_norm"] = camera.world_to_eye_norm
+    shadow = getattr(extra, "shadow", None)
+    if shadow is not None:
+        vals["shadow_strength"] = shadow.strength
+    for name, (o, n) in sorted(_SLICES.items(), key=lambda kv: kv[1][0]):
+        if o > pos:
+            parts.append(base[pos:o])
+        v = vals.get(name)
+        if v is None:
+            parts.append(base[o:o + n])
+        else:
+            t = _as_tensor(v, dev).float().reshape(-1)
+            if t.numel() == 1:
+                t = t.expand(n)
+            parts.append(t[:n])
+        pos = o + n
+    if pos < _hip.DSCENE_FLOATS:
+        parts.append(base[pos:])
+    return torch.cat(parts)[None, :]
+
+
+def render_differentiable(camera, shader, face_indices, extra, width: int, height: int, *,
+                          zbuffer_default: float = 1.0,
+                          colour_default=(1.0, 1.0, 1.0)) -> Tuple[torch.Tensor, torch.Tensor]:
+    """`render` into fresh buffers, differentiable in every tensor leaf of
+    `extra` and `camera` (see module docstring).  Returns (zbuffer [W, H],
+    canvas [W, H, 3])."""
+    if not is_builtin(shader):
+        raise NotImplementedError("render_differentiable supports the built-in shaders")
+    if shader._jr_shader_id == _hip.SHADER_PHONG_DARBOUX:
+        raise NotImplementedError("phong_darboux has no backward pass")
+    dev = _raster.require_device()
+
+    def host(x):
+        return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x
+
+    def host_tuple(nt):
+        return type(nt)(*(host(v) if not hasattr(v, "_fields") else host_tuple(v) for v in nt))
+
+    cam_h = type(camera)(*(host(v) for v in camera))
+    extra_h = host_tuple(extra)
+    call = build_call(cam_h, shader, host(face_indices), extra_h, width, height,
+                      init_buffers=True, zbuffer_default=zbuffer_default,
+                      colour_default=colour_default)
+    params = _row_tensor(dev, camera, extra, call)
+
+    def leaf(name):
+        v = getattr(extra, name, None)
+        return None if v is None else _as_tensor(v, dev).float()
+
+    zb, cv = RenderFunction.apply(call, params, leaf("position"), leaf("normal"), leaf("colour"),
+                                  leaf("texture"), leaf("specular_map"))
+    return zb[0], cv[0]
+
+
+__all__ = ["RenderFunction", "render_differentiable", "scene_rows", "scenes_with_rows"]

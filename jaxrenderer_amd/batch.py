@@ -258,5 +258,65 @@ class BatchRenderer:
             out.update(prim_id=main.prim_id, bary=main.bary)
         return out
 
+    # ---- reverse mode -----------------------------------------------------
+    def enable_backward(self) -> Dict[str, torch.Tensor]:
+        """Allocates the main pass's primitive-id buffer, the gradient buffers
+        (world-space vertices, normals, merged diffuse and specular maps, and
+        the per-scene parameter rows of include/jrhip.h's JR_DSCENE layout)
+        and the backward workspace.  The shadow pass needs no backward: the
+        shadow map only enters through comparisons."""
+        main = self.passes["main"]
+        dev = main.zbuffer.device
+        c = main.call
+        if main.prim_id is None:
+            main.prim_id = torch.empty((c.n_scenes, c.width, c.height), dtype=torch.int32,
+                                       device=dev)
+        self.grads = {
+            "d_positions": torch.zeros_like(main.inputs["positions"]),
+            "d_normals": torch.zeros_like(main.inputs["normals"]),
+            "d_textures": torch.zeros_like(main.inputs["textures"]),
+            "d_specular_maps": torch.zeros_like(main.inputs["specular_maps"]),
+            "d_scene": torch.zeros((c.n_scenes, _hip.DSCENE_FLOATS), dtype=torch.float32,
+                                   device=dev),
+        }
+        need = _raster.backward_workspace_size(width=c.width, height=c.height,
+                                               n_scenes=c.n_scenes, shader=c.shader,
+                                               n_tris=c.n_tris)
+        self.bwd_workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+        self.graph = None
+        return self.grads
+
+    def backward(self, d_canvas: torch.Tensor,
+                 d_zbuffer: Optional[torch.Tensor] = None) -> Dict[str, torch.Tensor]:
+        """Gradients of the last render() for upstream gradients of the canvas
+        [B, W, H, 3] (and optionally the zbuffer), written into self.grads."""
+        main = self.passes["main"]
+        c = main.call
+        for g in self.grads.values():
+            g.zero_()
+        _raster.backward(
+            shader=c.shader, width=c.width, height=c.height, scenes=main.scenes,
+            n_scenes=c.n_scenes, inputs=main.inputs, n_slots=c.n_slots, n_tris=c.n_tris,
+            prim_id=main.prim_id, d_canvas=d_canvas, d_zbuffer=d_zbuffer, grads=self.grads,
+            shadow_shape=c.shadow_shape, workspace=self.bwd_workspace)
+        return self.grads
+
+    def train_step(self, target: torch.Tensor) -> Dict[str, torch.Tensor]:
+        """Forward, the gradient of the L2 image loss sum((canvas - target)^2),
+        and backward."""
+        out = self.render()
+        self._d_canvas = 2.0 * (out["canvas"] - target)
+        return self.backward(self._d_canvas)
+
+    def capture_train(self, target: torch.Tensor) -> torch.cuda.CUDAGraph:
+        """Captures train_step(target) in one HIP graph."""
+        self.train_step(target)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.train_step(target)
+        self.train_graph = g
+        return g
+
 
 __all__ = ["BatchRenderer", "BatchScene", "ShadowMapSpec", "plan_batch"]

@@ -1,9 +1,717 @@
-// jr_backward.hip -- reverse-mode pass (placeholder until the kernels land).
+// jr_backward.hip -- reverse-mode pass of the forward rasteriser (gfx950).
+//
+// What jax.grad computes through renderer/pipeline.py with visibility held
+// piecewise constant: the chosen primitive, the inside test, keep flags,
+// floor/round texture and shadow lookups and all comparisons carry no
+// gradient; gradients flow through the barycentrics b = e / sum(e) with
+// e = (x, y, 1) M^-1 (so through jnp.linalg.inv: dM = -M^-T G M^-T), the
+// window depth, the interpolated varyings, the shading formulas, texture and
+// specular-map values (a gather transposes to a scatter-add at the same
+// clamped index), the light and the camera matrices.  JAX's conventions are
+// kept: max(a, b) splits the gradient 1/2-1/2 on ties, pow(x, y) has
+// d/dy = log(x) x^y with log(0) read as log(1), and d/dx = 0 when y == 0.
+//
+// Nothing is stored from the forward but prim_id: for every written pixel the
+// kernels recompute the chosen primitive's setup, edge values, barycentrics
+// and fragment attributes with the forward's own code (jr_common.h,
+// jr_shade.h), so they are bit-identical to what the forward used.
+//
+//   k_bwd_clear   zero the per-triangle accumulators
+//   k_bwd_pixels  one 64-lane group per 8x8 tile: per pixel, shading backward
+//                 (texture / specular scatter, per-scene reductions) and the
+//                 visibility backward (dL/dM^-1, dL/dz_clip, per-corner varying
+//                 cotangents), grouped per triangle in LDS before one set of
+//                 global atomics per (tile, triangle)
+//   k_bwd_tris    one thread per triangle: M^-1 -> M -> clip -> positions and
+//                 the vertex stage of each corner (normals, colours, light,
+//                 camera matrices)
 #include <hip/hip_runtime.h>
-#include "../../include/jrhip.h"
 
-extern "C" int32_t jr_backward(const jr_frame*, const jr_inputs*, const jr_scene*,
-                               const float*, const float*, const int32_t*,
-                               const float*, jr_grads*, jr_stream_t) {
-  return JR_E_UNSUPPORTED;
+#include <algorithm>
+#include <string>
+
+#include "../../include/jrhip.h"
+#include "jr_common.h"
+#include "jr_math.h"
+#include "jr_shade.h"
+
+namespace jr {
+
+constexpr int kAcc = 21;   // dMinv[3][3] (row r, column k), dz_clip[3], dvar[3][3]
+constexpr int A_MINV = 0;
+constexpr int A_Z = 9;
+constexpr int A_VAR = 12;
+
+// per-scene gradients reduced in k_bwd_pixels
+enum {
+  S_LIGHT_COLOUR = 0,
+  S_LIGHT_DIR_EYE = 3,
+  S_AMBIENT = 6,
+  S_DIFFUSE = 9,
+  S_SPECULAR = 12,
+  S_STRENGTH = 15,
+  S_LIGHT_DIR = 18,
+  S_VP00 = 21,
+  S_VP03 = 22,
+  S_VP11 = 23,
+  S_VP13 = 24,
+  S_VP22 = 25,
+  S_VP23 = 26,
+  S_COUNT = 27
+};
+
+__device__ __constant__ int kSceneSlot[S_COUNT] = {
+    JR_DSCENE_LIGHT_COLOUR, JR_DSCENE_LIGHT_COLOUR + 1, JR_DSCENE_LIGHT_COLOUR + 2,
+    JR_DSCENE_LIGHT_DIR_EYE, JR_DSCENE_LIGHT_DIR_EYE + 1, JR_DSCENE_LIGHT_DIR_EYE + 2,
+    JR_DSCENE_AMBIENT, JR_DSCENE_AMBIENT + 1, JR_DSCENE_AMBIENT + 2,
+    JR_DSCENE_DIFFUSE, JR_DSCENE_DIFFUSE + 1, JR_DSCENE_DIFFUSE + 2,
+    JR_DSCENE_SPECULAR, JR_DSCENE_SPECULAR + 1, JR_DSCENE_SPECULAR + 2,
+    JR_DSCENE_SHADOW_STRENGTH, JR_DSCENE_SHADOW_STRENGTH + 1, JR_DSCENE_SHADOW_STRENGTH + 2,
+    JR_DSCENE_LIGHT_DIRECTION, JR_DSCENE_LIGHT_DIRECTION + 1, JR_DSCENE_LIGHT_DIRECTION + 2,
+    JR_DSCENE_VIEWPORT + 0, JR_DSCENE_VIEWPORT + 3, JR_DSCENE_VIEWPORT + 5,
+    JR_DSCENE_VIEWPORT + 7, JR_DSCENE_VIEWPORT + 10, JR_DSCENE_VIEWPORT + 11};
+
+// y = v / |v|: dv = (dy - y (y . dy)) / |v|
+JR_DEV void normalise_bwd(const float v[3], const float dy[3], float dv[3]) {
+  float ss = v[0] * v[0];
+  ss = ss + v[1] * v[1];
+  ss = ss + v[2] * v[2];
+  const float n = sqrtf(ss);
+  const float y[3] = {v[0] / n, v[1] / n, v[2] / n};
+  const float yd = dot3(y, dy);
+  for (int q = 0; q < 3; ++q) dv[q] = (dy[q] - y[q] * yd) / n;
 }
+
+// d max(a, b) / da under JAX's balanced rule
+JR_DEV float max_grad(float a, float b) {
+  return a > b ? 1.0f : (a == b ? 0.5f : 0.0f);
+}
+
+JR_DEV void add_tex(float* __restrict__ d, int64_t base, int32_t th, int32_t tc, int32_t u,
+                    int32_t v, const float g[3]) {
+  if (!d) return;
+  const int64_t o = base + ((int64_t)u * th + v) * tc;
+  if (tc > 1) {
+    atomicAdd(d + o, g[0]);
+    atomicAdd(d + o + 1, g[1]);
+    atomicAdd(d + o + 2, g[2]);
+  } else {
+    atomicAdd(d + o, (g[0] + g[1]) + g[2]);
+  }
+}
+
+struct PixGrad {
+  float db[3];
+  float dvar[3][3];
+  float sg[S_COUNT];
+};
+
+template <int SHADER>
+struct ShadeBwd;
+
+template <>
+struct ShadeBwd<JR_SHADER_DEPTH> {
+  static JR_DEV void run(const jr_inputs&, const jr_scene&, const jr_grads&, const Corner&,
+                         const float*, const float*, PixGrad&, int) {}
+};
+
+// shaders/gouraud.py:90-124: col = sum_k b_k col_k
+template <>
+struct ShadeBwd<JR_SHADER_GOURAUD> {
+  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads&,
+                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
+                         int) {
+    float v[3][3];
+    for (int k = 0; k < 3; ++k) gouraud_vertex(in, sc, c.slot[k], v[k]);
+    for (int k = 0; k < 3; ++k) {
+      g.db[k] = dot3(dcol, v[k]);
+      for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dcol[q];
+    }
+  }
+};
+
+// shaders/gouraud_texture.py:103-144: col = texture[uv] * sum_k b_k lc_k
+template <>
+struct ShadeBwd<JR_SHADER_GOURAUD_TEXTURE> {
+  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
+                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
+                         int) {
+    float lc[3][3], uvv[3][2], ln[3];
+    normalise3(sc.light_direction, ln);
+    for (int k = 0; k < 3; ++k) {
+      float n[3], nn[3];
+      load3(in.normals, attr_row(in.nrm_index, c.slot[k]), n);
+      normalise3(n, nn);
+      const float it = dot3(nn, ln);
+      for (int q = 0; q < 3; ++q) lc[k][q] = sc.light_colour[q] * it;
+      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
+    }
+    float light[3];
+    for (int q = 0; q < 3; ++q) light[q] = interp(b, lc[0][q], lc[1][q], lc[2][q]);
+    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
+    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
+    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    float tex[3];
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
+    float dt[3], dl[3];
+    for (int q = 0; q < 3; ++q) {
+      dt[q] = dcol[q] * light[q];
+      dl[q] = dcol[q] * tex[q];
+    }
+    add_tex(gr.d_textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    for (int k = 0; k < 3; ++k) {
+      g.db[k] = dot3(dl, lc[k]);
+      for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dl[q];
+    }
+  }
+};
+
+// shaders/phong.py:106-164
+template <>
+struct ShadeBwd<JR_SHADER_PHONG> {
+  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
+                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
+                         int) {
+    float ne[3][3], uvv[3][2];
+    for (int k = 0; k < 3; ++k) {
+      eye_normal(in, sc, c.slot[k], ne[k]);
+      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
+    }
+    float n[3];
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
+    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
+    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
+    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    float tex[3], nn[3], ln[3];
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
+    normalise3(n, nn);
+    normalise3(sc.light_direction, ln);
+    const float d = dot3(nn, ln);
+    float lc[3];
+    bool pos = true;
+    for (int q = 0; q < 3; ++q) {
+      lc[q] = sc.light_colour[q] * d;
+      pos = pos && (lc[q] >= 0.0f);
+    }
+    if (!pos) return;  // the zero branch of the cond carries no gradient
+    float dt[3], dlc[3];
+    float dd = 0.0f;
+    for (int q = 0; q < 3; ++q) {
+      dt[q] = dcol[q] * lc[q];
+      dlc[q] = dcol[q] * tex[q];
+      g.sg[S_LIGHT_COLOUR + q] += dlc[q] * d;
+      dd += dlc[q] * sc.light_colour[q];
+    }
+    add_tex(gr.d_textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    float dnn[3], dln[3], dn[3], dL[3];
+    for (int q = 0; q < 3; ++q) {
+      dnn[q] = dd * ln[q];
+      dln[q] = dd * nn[q];
+    }
+    normalise_bwd(sc.light_direction, dln, dL);
+    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR + q] += dL[q];
+    normalise_bwd(n, dnn, dn);
+    for (int k = 0; k < 3; ++k) {
+      g.db[k] = dot3(dn, ne[k]);
+      for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dn[q];
+    }
+  }
+};
+
+// shaders/phong_reflection.py:154-235 and phong_reflection_shadow.py:173-271
+template <bool SHADOW>
+struct ReflectionBwd {
+  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
+                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
+                         int scene) {
+    float ne[3][3], uvv[3][2], scd[3][4];
+    for (int k = 0; k < 3; ++k) {
+      eye_normal(in, sc, c.slot[k], ne[k]);
+      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
+      if (SHADOW) {
+        float h[4];
+        position_clip(in, sc.shadow_world_to_clip, c.slot[k], h);
+        for (int q = 0; q < 4; ++q) scd[k][q] = h[q] / h[3];
+      }
+    }
+    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+    float n[3], uv[2];
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
+    uv[0] = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
+    uv[1] = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    bool lit = true;
+    if (SHADOW) {
+      float s4[4], sv[4];
+      for (int q = 0; q < 4; ++q) s4[q] = interp(b, scd[0][q], scd[1][q], scd[2][q]);
+      apply4(sc.shadow_viewport, s4[0], s4[1], s4[2], s4[3], sv);
+      const float* map = in.shadow_maps + (int64_t)scene * in.shadow_w * in.shadow_h;
+      lit = (sv[2] / sv[3]) <= shadow_get(map, in.shadow_w, in.shadow_h, sv[0] / sv[3],
+                                          sv[1] / sv[3]);
+    }
+    // forward of reflection_terms with its intermediates
+    const int32_t row = index_clamp(ti, sc.texshape_count);
+    const int32_t s0 = in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 0];
+    const int32_t s1 = in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 1];
+    float f0 = modf_frac(uv[0]), f1 = modf_frac(uv[1]);
+    f0 = (f0 < 0.0f) ? f0 + 1.0f : f0;
+    f1 = (f1 < 0.0f) ? f1 + 1.0f : f1;
+    const int32_t shift = (int32_t)((uint32_t)ti * (uint32_t)sc.tex_offset);
+    const int32_t u0 = f2i_sat(floorf(f0 * (float)s0 + (float)shift));
+    const int32_t v0 = f2i_sat(floorf(f1 * (float)s1));
+    const int32_t u = index_clamp(u0, sc.tex_w), v = index_clamp(v0, sc.tex_h);
+    const int32_t us = index_clamp(u0, sc.spec_w), vs = index_clamp(v0, sc.spec_h);
+    float tex[3];
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
+    const int64_t spec_at = sc.spec_base + (int64_t)us * sc.spec_h + vs;
+    const float S = in.specular_maps[spec_at];
+    float nh[3], lh[3];
+    normalise3(n, nh);
+    normalise3(sc.light_dir_eye, lh);
+    const float d = dot3(nh, lh);
+    const float diff = max_nan(d, 0.0f);
+    const float two_d = 2.0f * d;
+    float qv[3] = {two_d * nh[0] - lh[0], two_d * nh[1] - lh[1], two_d * nh[2] - lh[2]};
+    float rh[3];
+    normalise3(qv, rh);
+    const float m = max_nan(rh[2], 0.0f);
+    const float spec = powf(m, S);
+
+    float dt[3], ddiff = 0.0f, dspec = 0.0f;
+    for (int q = 0; q < 3; ++q) {
+      const float k = sc.diffuse[q] * diff + sc.specular[q] * spec;
+      const float sh = lit ? 1.0f : 1.0f - sc.shadow_strength[q];
+      g.sg[S_AMBIENT + q] += dcol[q] * tex[q];
+      float dk;
+      if (SHADOW) {
+        dt[q] = dcol[q] * (sc.ambient[q] + (sh * k) * sc.light_colour[q]);
+        dk = dcol[q] * sh * tex[q] * sc.light_colour[q];
+        g.sg[S_LIGHT_COLOUR + q] += dcol[q] * (sh * k) * tex[q];
+        if (!lit) g.sg[S_STRENGTH + q] += -dcol[q] * k * tex[q] * sc.light_colour[q];
+      } else {
+        dt[q] = dcol[q] * (sc.ambient[q] + k * sc.light_colour[q]);
+        dk = dcol[q] * sc.light_colour[q] * tex[q];
+        g.sg[S_LIGHT_COLOUR + q] += dcol[q] * k * tex[q];
+      }
+      g.sg[S_DIFFUSE + q] += dk * diff;
+      g.sg[S_SPECULAR + q] += dk * spec;
+      ddiff += dk * sc.diffuse[q];
+      dspec += dk * sc.specular[q];
+    }
+    add_tex(gr.d_textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    if (gr.d_specular_maps) {
+      const float lg = logf(m == 0.0f ? 1.0f : m);
+      atomicAdd(gr.d_specular_maps + spec_at, dspec * lg * spec);
+    }
+    const float dm = dspec * (S == 0.0f ? 0.0f : S * powf(m, S - 1.0f));
+    const float dr[3] = {0.0f, 0.0f, dm * max_grad(rh[2], 0.0f)};
+    float dq[3];
+    normalise_bwd(qv, dr, dq);
+    float dnh[3], dlh[3];
+    float dtwo_d = dot3(dq, nh);
+    for (int q = 0; q < 3; ++q) {
+      dnh[q] = two_d * dq[q];
+      dlh[q] = -dq[q];
+    }
+    const float dd = 2.0f * dtwo_d + ddiff * max_grad(d, 0.0f);
+    for (int q = 0; q < 3; ++q) {
+      dnh[q] += dd * lh[q];
+      dlh[q] += dd * nh[q];
+    }
+    float dn[3], dle[3];
+    normalise_bwd(n, dnh, dn);
+    normalise_bwd(sc.light_dir_eye, dlh, dle);
+    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR_EYE + q] += dle[q];
+    for (int k = 0; k < 3; ++k) {
+      g.db[k] = dot3(dn, ne[k]);
+      for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dn[q];
+    }
+  }
+};
+
+template <>
+struct ShadeBwd<JR_SHADER_PHONG_REFLECTION> : ReflectionBwd<false> {};
+template <>
+struct ShadeBwd<JR_SHADER_PHONG_REFLECTION_SHADOW> : ReflectionBwd<true> {};
+
+JR_DEV float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+struct BwdArgs {
+  jr_inputs in;
+  const jr_scene* scenes;
+  jr_grads gr;
+  const float* d_canvas;
+  const float* d_zbuffer;
+  const int32_t* prim_id;
+  float* tacc;
+  int32_t W, H, tiles_x, tiles_y, T, n_scenes;
+};
+
+template <int SHADER>
+__global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
+  __shared__ float acc[64][kAcc + 1];
+  const int lane = threadIdx.x;
+  const int64_t gt = blockIdx.x;
+  const int32_t s = (int32_t)(gt / a.T);
+  const int32_t tile = (int32_t)(gt - (int64_t)s * a.T);
+  const int32_t tx = tile / a.tiles_y, ty = tile - tx * a.tiles_y;
+  const int32_t i = tx * 8 + lane / 8, j = ty * 8 + lane % 8;
+  const jr_scene& sc = a.scenes[s];
+  const float* vp = sc.viewport;
+  const int64_t pix = (int64_t)s * a.W * a.H + (int64_t)i * a.H + j;
+  int32_t t = -1;
+  if (i < a.W && j < a.H) t = a.prim_id[pix];
+
+  PixGrad g;
+  for (int q = 0; q < S_COUNT; ++q) g.sg[q] = 0.0f;
+  float vals[kAcc];
+  for (int q = 0; q < kAcc; ++q) vals[q] = 0.0f;
+  if (t >= 0) {
+    TriSetup ts;
+    tri_setup(a.in, sc, t, ts);
+    const float4 r0 = rec_col(ts, 0), r1 = rec_col(ts, 1), r2 = rec_col(ts, 2);
+    const float x = pix_x(i, vp), y = pix_y(j, vp);
+    const float e[3] = {edge(r0, x, y), edge(r1, x, y), edge(r2, x, y)};
+    float wr = e[0] + e[1];
+    wr = wr + e[2];
+    const float b[3] = {e[0] / wr, e[1] / wr, e[2] / wr};
+    for (int k = 0; k < 3; ++k) {
+      g.db[k] = 0.0f;
+      for (int q = 0; q < 3; ++q) g.dvar[k][q] = 0.0f;
+    }
+    Corner c;
+    c.slot[0] = a.in.faces[3 * (int64_t)t + 0];
+    c.slot[1] = a.in.faces[3 * (int64_t)t + 1];
+    c.slot[2] = a.in.faces[3 * (int64_t)t + 2];
+    if (a.d_canvas) {
+      const float dcol[3] = {a.d_canvas[3 * pix], a.d_canvas[3 * pix + 1],
+                             a.d_canvas[3 * pix + 2]};
+      ShadeBwd<SHADER>::run(a.in, sc, a.gr, c, b, dcol, g, s);
+    }
+    // b = e / sum(e)
+    const float tdb = (g.db[0] * e[0] + g.db[1] * e[1]) + g.db[2] * e[2];
+    float de[3];
+    for (int k = 0; k < 3; ++k) de[k] = g.db[k] / wr - tdb / (wr * wr);
+    // zbuffer = (sum_k e_k z_k) * vp22 + vp23
+    if (a.d_zbuffer) {
+      const float dz = a.d_zbuffer[pix];
+      float zc = e[0] * r0.w;
+      zc = zc + e[1] * r1.w;
+      zc = zc + e[2] * r2.w;
+      const float zk[3] = {r0.w, r1.w, r2.w};
+      for (int k = 0; k < 3; ++k) {
+        de[k] += dz * vp[10] * zk[k];
+        vals[A_Z + k] = dz * vp[10] * e[k];
+      }
+      g.sg[S_VP22] += dz * zc;
+      g.sg[S_VP23] += dz;
+    }
+    // e_k = x Minv[0][k] + y Minv[1][k] + Minv[2][k]
+    float dx = 0.0f, dy = 0.0f;
+    const float m0[3] = {r0.x, r1.x, r2.x}, m1[3] = {r0.y, r1.y, r2.y};
+    for (int k = 0; k < 3; ++k) {
+      vals[A_MINV + 0 * 3 + k] = x * de[k];
+      vals[A_MINV + 1 * 3 + k] = y * de[k];
+      vals[A_MINV + 2 * 3 + k] = de[k];
+      dx += m0[k] * de[k];
+      dy += m1[k] * de[k];
+    }
+    // x = (i - vp03) / vp00, y = (j - vp13) / vp11
+    g.sg[S_VP03] += -dx / vp[0];
+    g.sg[S_VP00] += -dx * x / vp[0];
+    g.sg[S_VP13] += -dy / vp[5];
+    g.sg[S_VP11] += -dy * y / vp[5];
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < 3; ++q) vals[A_VAR + 3 * k + q] = g.dvar[k][q];
+  }
+
+  // group the lanes by triangle: leader = lowest lane with the same id
+  int leader = lane;
+  uint64_t pending = __ballot(t >= 0);
+  while (pending) {
+    const int l = __ffsll((unsigned long long)pending) - 1;
+    const int32_t id = __shfl(t, l);
+    const uint64_t same = __ballot(t == id);
+    if (t == id) leader = l;
+    pending &= ~same;
+  }
+  for (int q = 0; q < kAcc; ++q) acc[lane][q] = 0.0f;
+  __syncthreads();
+  if (t >= 0)
+    for (int q = 0; q < kAcc; ++q)
+      if (vals[q] != 0.0f) atomicAdd(&acc[leader][q], vals[q]);
+  __syncthreads();
+  if (t >= 0 && leader == lane)
+    for (int q = 0; q < kAcc; ++q) {
+      const float v = acc[lane][q];
+      if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + q, v);
+    }
+  // per-scene parameters: one reduction per tile
+  if (a.gr.d_scene) {
+    for (int q = 0; q < S_COUNT; ++q) {
+      const float v = wave_sum(g.sg[q]);
+      if (lane == 0 && v != 0.0f)
+        atomicAdd(a.gr.d_scene + (int64_t)s * JR_DSCENE_FLOATS + kSceneSlot[q], v);
+    }
+  }
+}
+
+// vertex-stage backward of one slot for its varying cotangent dv
+template <int SHADER>
+JR_DEV void vertex_bwd(const BwdArgs& a, const jr_scene& sc, int32_t slot, const float dv[3],
+                       float* dW, float* dE, float* dLd, float* dLc) {
+  if (SHADER == JR_SHADER_DEPTH) return;
+  if (dv[0] == 0.0f && dv[1] == 0.0f && dv[2] == 0.0f) return;
+  const jr_inputs& in = a.in;
+  const int32_t nrow = attr_row(in.nrm_index, slot);
+  float n[3], nn[3];
+  load3(in.normals, nrow, n);
+  normalise3(n, nn);
+  float dnn[3];
+  if (SHADER == JR_SHADER_GOURAUD || SHADER == JR_SHADER_GOURAUD_TEXTURE) {
+    float ln[3];
+    normalise3(sc.light_direction, ln);
+    const float it = dot3(nn, ln);
+    float dI = 0.0f;
+    if (SHADER == JR_SHADER_GOURAUD) {
+      const int32_t crow = attr_row(in.col_index, slot);
+      float col[3];
+      load3(in.colours, crow, col);
+      for (int q = 0; q < 3; ++q) {
+        if (a.gr.d_colours)
+          atomicAdd(a.gr.d_colours + 3 * (int64_t)crow + q, dv[q] * sc.light_colour[q] * it);
+        dLc[q] += dv[q] * col[q] * it;
+        dI += dv[q] * col[q] * sc.light_colour[q];
+      }
+    } else {
+      for (int q = 0; q < 3; ++q) {
+        dLc[q] += dv[q] * it;
+        dI += dv[q] * sc.light_colour[q];
+      }
+    }
+    float dln[3], dL[3];
+    for (int q = 0; q < 3; ++q) {
+      dnn[q] = dI * ln[q];
+      dln[q] = dI * nn[q];
+    }
+    normalise_bwd(sc.light_direction, dln, dL);
+    for (int q = 0; q < 3; ++q) dLd[q] += dL[q];
+  } else {
+    // ne = normalise((E @ [normalise(nn), 0])[:3])  (Camera.apply_vec)
+    float u2[3];
+    normalise3(nn, u2);
+    float h[4];
+    apply4(sc.world_to_eye_norm, u2[0], u2[1], u2[2], 0.0f, h);
+    const float t3[3] = {h[0], h[1], h[2]};
+    float dt3[3], du2[3], du1[3];
+    normalise_bwd(t3, dv, dt3);
+    for (int jj = 0; jj < 3; ++jj) {
+      float s = 0.0f;
+      for (int ii = 0; ii < 3; ++ii) {
+        s += sc.world_to_eye_norm[4 * ii + jj] * dt3[ii];
+        dE[4 * ii + jj] += dt3[ii] * u2[jj];
+      }
+      du2[jj] = s;
+    }
+    normalise_bwd(nn, du2, du1);
+    for (int q = 0; q < 3; ++q) dnn[q] = du1[q];
+  }
+  float dn[3];
+  normalise_bwd(n, dnn, dn);
+  if (a.gr.d_normals)
+    for (int q = 0; q < 3; ++q) atomicAdd(a.gr.d_normals + 3 * (int64_t)nrow + q, dn[q]);
+}
+
+template <int SHADER>
+__global__ __launch_bounds__(256) void k_bwd_tris(BwdArgs a) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  float dW[16], dE[16], dLd[3], dLc[3];
+  for (int q = 0; q < 16; ++q) dW[q] = dE[q] = 0.0f;
+  for (int q = 0; q < 3; ++q) dLd[q] = dLc[q] = 0.0f;
+  int32_t s = -1;
+  if (t < a.in.n_tris) {
+    const float* A = a.tacc + (int64_t)t * kAcc;
+    bool any = false;
+    float v[kAcc];
+    for (int q = 0; q < kAcc; ++q) {
+      v[q] = A[q];
+      any = any || (v[q] != 0.0f);
+    }
+    s = scene_of_tri(a.scenes, a.n_scenes, t);
+    if (any) {
+      const jr_scene& sc = a.scenes[s];
+      TriSetup ts;
+      tri_setup(a.in, sc, t, ts);
+      // X = M^-1, G = dL/dX: dL/dM = -X^T G X^T
+      const float* X = ts.Mi;
+      float XtG[9];
+      for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 3; ++k) {
+          float s2 = 0.0f;
+          for (int r = 0; r < 3; ++r) s2 += X[3 * r + p] * v[A_MINV + 3 * r + k];
+          XtG[3 * p + k] = s2;
+        }
+      float dM[9];
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+          float s2 = 0.0f;
+          for (int k = 0; k < 3; ++k) s2 += XtG[3 * p + k] * X[3 * q + k];
+          dM[3 * p + q] = -s2;
+        }
+      for (int k = 0; k < 3; ++k) {
+        const int32_t slot = a.in.faces[3 * (int64_t)t + k];
+        const float dclip[4] = {dM[3 * k + 0], dM[3 * k + 1], v[A_Z + k], dM[3 * k + 2]};
+        const int32_t prow = attr_row(a.in.pos_index, slot);
+        float p[3];
+        load3(a.in.positions, prow, p);
+        const float ph[4] = {p[0], p[1], p[2], 1.0f};
+        for (int jj = 0; jj < 4; ++jj) {
+          float s2 = 0.0f;
+          for (int ii = 0; ii < 4; ++ii) {
+            s2 += sc.world_to_clip[4 * ii + jj] * dclip[ii];
+            dW[4 * ii + jj] += dclip[ii] * ph[jj];
+          }
+          if (jj < 3 && a.gr.d_positions) atomicAdd(a.gr.d_positions + 3 * (int64_t)prow + jj, s2);
+        }
+        const float dv[3] = {v[A_VAR + 3 * k], v[A_VAR + 3 * k + 1], v[A_VAR + 3 * k + 2]};
+        vertex_bwd<SHADER>(a, sc, slot, dv, dW, dE, dLd, dLc);
+      }
+    }
+  }
+  if (!a.gr.d_scene) return;
+  // reduce over the wave when it holds one scene, else per lane
+  const int32_t s0 = __shfl(s, 0);
+  const bool uniform = __ballot(s == s0 || s < 0) == __ballot(true) && s0 >= 0;
+  float* base = a.gr.d_scene;
+  if (uniform) {
+    const int lane = threadIdx.x & 63;
+    float* row = base + (int64_t)s0 * JR_DSCENE_FLOATS;
+    for (int q = 0; q < 16; ++q) {
+      const float w = wave_sum(dW[q]);
+      const float e = wave_sum(dE[q]);
+      if (lane == 0) {
+        if (w != 0.0f) atomicAdd(row + JR_DSCENE_WORLD_TO_CLIP + q, w);
+        if (e != 0.0f) atomicAdd(row + JR_DSCENE_WORLD_TO_EYE_NORM + q, e);
+      }
+    }
+    for (int q = 0; q < 3; ++q) {
+      const float l = wave_sum(dLd[q]);
+      const float c = wave_sum(dLc[q]);
+      if (lane == 0) {
+        if (l != 0.0f) atomicAdd(row + JR_DSCENE_LIGHT_DIRECTION + q, l);
+        if (c != 0.0f) atomicAdd(row + JR_DSCENE_LIGHT_COLOUR + q, c);
+      }
+    }
+  } else if (s >= 0) {
+    float* row = base + (int64_t)s * JR_DSCENE_FLOATS;
+    for (int q = 0; q < 16; ++q) {
+      if (dW[q] != 0.0f) atomicAdd(row + JR_DSCENE_WORLD_TO_CLIP + q, dW[q]);
+      if (dE[q] != 0.0f) atomicAdd(row + JR_DSCENE_WORLD_TO_EYE_NORM + q, dE[q]);
+    }
+    for (int q = 0; q < 3; ++q) {
+      if (dLd[q] != 0.0f) atomicAdd(row + JR_DSCENE_LIGHT_DIRECTION + q, dLd[q]);
+      if (dLc[q] != 0.0f) atomicAdd(row + JR_DSCENE_LIGHT_COLOUR + q, dLc[q]);
+    }
+  }
+}
+
+__global__ void k_bwd_clear(float* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+thread_local std::string g_bwd_err;
+
+int32_t bfail(int32_t code, const char* msg) {
+  g_bwd_err = msg;
+  return code;
+}
+
+template <int SHADER>
+void launch_bwd(const BwdArgs& a, hipStream_t st) {
+  const int64_t n_tiles = (int64_t)a.T * a.n_scenes;
+  hipLaunchKernelGGL(k_bwd_pixels<SHADER>, dim3((unsigned)n_tiles), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(k_bwd_tris<SHADER>, dim3((a.in.n_tris + 255) / 256), dim3(256), 0, st, a);
+}
+
+}  // namespace jr
+
+using namespace jr;
+
+extern "C" {
+
+int64_t jr_backward_workspace_size(const jr_frame* frame, int32_t n_tris) {
+  if (!frame || n_tris < 0) return -1;
+  return ((4 * (int64_t)kAcc * std::max(n_tris, 1)) + 255) / 256 * 256;
+}
+
+const char* jr_backward_last_error(void) { return g_bwd_err.c_str(); }
+
+int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                    const float* d_canvas, const float* d_zbuffer, const int32_t* prim_id,
+                    jr_grads* grads, void* workspace, int64_t workspace_bytes,
+                    jr_stream_t stream_) {
+  if (!fr || !in || !scenes || !grads || !prim_id)
+    return bfail(JR_E_INVALID, "frame, inputs, scenes, grads and prim_id are required");
+  if (fr->width <= 0 || fr->height <= 0 || fr->n_scenes <= 0 || in->n_tris < 0)
+    return bfail(JR_E_INVALID, "bad frame sizes");
+  if (in->clip) return bfail(JR_E_UNSUPPORTED, "backward needs positions, not clip input");
+  const int sh = fr->shader;
+  if (sh == JR_SHADER_PHONG_DARBOUX || sh == JR_SHADER_VISIBILITY || sh < 0 ||
+      sh >= JR_SHADER_COUNT)
+    return bfail(JR_E_UNSUPPORTED, "shader has no backward pass");
+  if (!d_canvas && !d_zbuffer) return bfail(JR_E_INVALID, "no upstream gradient");
+  if (!in->positions || !in->faces) return bfail(JR_E_INVALID, "positions and faces are required");
+  if (sh == JR_SHADER_GOURAUD && (!in->normals || !in->colours))
+    return bfail(JR_E_INVALID, "gouraud needs normals and colours");
+  if ((sh == JR_SHADER_GOURAUD_TEXTURE || sh == JR_SHADER_PHONG ||
+       sh == JR_SHADER_PHONG_REFLECTION || sh == JR_SHADER_PHONG_REFLECTION_SHADOW) &&
+      (!in->normals || !in->uvs || !in->textures))
+    return bfail(JR_E_INVALID, "textured shaders need normals, uvs and textures");
+  if ((sh == JR_SHADER_PHONG_REFLECTION || sh == JR_SHADER_PHONG_REFLECTION_SHADOW) &&
+      (!in->tex_index || !in->texture_shapes || !in->specular_maps))
+    return bfail(JR_E_INVALID, "phong_reflection needs tex_index, texture_shapes, specular_maps");
+  if (sh == JR_SHADER_PHONG_REFLECTION_SHADOW &&
+      (!in->shadow_maps || in->shadow_w <= 0 || in->shadow_h <= 0))
+    return bfail(JR_E_INVALID, "phong_reflection_shadow needs shadow maps");
+  const int64_t need = jr_backward_workspace_size(fr, in->n_tris);
+  if (!workspace || workspace_bytes < need) return bfail(JR_E_WORKSPACE, "workspace too small");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream_);
+  BwdArgs a;
+  a.in = *in;
+  a.scenes = scenes;
+  a.gr = *grads;
+  a.d_canvas = (sh == JR_SHADER_DEPTH) ? nullptr : d_canvas;
+  a.d_zbuffer = d_zbuffer;
+  a.prim_id = prim_id;
+  a.tacc = static_cast<float*>(workspace);
+  a.W = fr->width;
+  a.H = fr->height;
+  a.tiles_x = (a.W + 7) / 8;
+  a.tiles_y = (a.H + 7) / 8;
+  a.T = a.tiles_x * a.tiles_y;
+  a.n_scenes = fr->n_scenes;
+  if (in->n_tris == 0) return JR_OK;
+  const int64_t nacc = (int64_t)kAcc * in->n_tris;
+  hipLaunchKernelGGL(k_bwd_clear, dim3((unsigned)std::min<int64_t>((nacc + 255) / 256, 8192)),
+                     dim3(256), 0, st, a.tacc, nacc);
+  switch (sh) {
+    case JR_SHADER_DEPTH: launch_bwd<JR_SHADER_DEPTH>(a, st); break;
+    case JR_SHADER_GOURAUD: launch_bwd<JR_SHADER_GOURAUD>(a, st); break;
+    case JR_SHADER_GOURAUD_TEXTURE: launch_bwd<JR_SHADER_GOURAUD_TEXTURE>(a, st); break;
+    case JR_SHADER_PHONG: launch_bwd<JR_SHADER_PHONG>(a, st); break;
+    case JR_SHADER_PHONG_REFLECTION: launch_bwd<JR_SHADER_PHONG_REFLECTION>(a, st); break;
+    default: launch_bwd<JR_SHADER_PHONG_REFLECTION_SHADOW>(a, st); break;
+  }
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return bfail(JR_E_HIP, hipGetErrorString(err));
+  return JR_OK;
+}
+
+}  // extern "C"

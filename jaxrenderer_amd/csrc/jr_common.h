@@ -1,0 +1,114 @@
+// jr_common.h -- per-triangle setup and per-pixel visibility arithmetic shared
+// by the forward and backward kernels, so both evaluate the reference's
+// formulas (renderer/pipeline.py:76-113, 170-279) with the very same code.
+#pragma once
+
+#include "../../include/jrhip.h"
+#include "jr_math.h"
+#include "jr_shade.h"
+
+namespace jr {
+
+// ---------------------------------------------------------------------------
+// triangle records
+//
+// rec[4t+k] for k < 3 is column k of M^-1 with the clip-space z of vertex k:
+// (Minv[0][k], Minv[1][k], Minv[2][k], z_k), so the edge value is
+// e_k = (x * r.x + y * r.y) + r.z.  rec[4t+3] = (det, flags, 0, 0).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kKeep = 1u;     // |det| > 1e-6
+constexpr uint32_t kFront = 2u;    // det >= 0
+constexpr uint32_t kSane = 4u;     // all coefficients finite and < 1e18
+
+struct Rect {
+  int32_t tx0, tx1, ty0, ty1;  // inclusive range, empty when tx0 > tx1
+};
+
+// NDC sample position of pixel (i, j) (renderer/pipeline.py:177).
+JR_DEV float pix_x(int32_t i, const float* vp) {
+  return ((float)i - vp[3]) / vp[0];
+}
+JR_DEV float pix_y(int32_t j, const float* vp) {
+  return ((float)j - vp[7]) / vp[5];
+}
+
+// Edge value e_k of the reference: dot((x, y, 1), Minv[:, k]).
+JR_DEV float edge(const float4& r, float x, float y) {
+  float e = x * r.x;
+  e = e + y * r.y;
+  e = e + r.z;
+  return e;
+}
+
+// Window-space depth: interpolate(z_clip, e) * vp[2][2] + vp[2][3]
+// (renderer/pipeline.py:212-219).
+JR_DEV float win_depth(const float4& r0, const float4& r1, const float4& r2,
+                       float e0, float e1, float e2, float vp22, float vp23) {
+  float z = e0 * r0.w;
+  z = z + e1 * r1.w;
+  z = z + e2 * r2.w;
+  return z * vp22 + vp23;
+}
+
+struct Frame {
+  int32_t W, H, TW, TH, tiles_x, tiles_y, T;
+  int32_t n_scenes;
+  int64_t capacity;
+};
+
+JR_DEV int32_t scene_of_tri(const jr_scene* __restrict__ scenes, int32_t n,
+                            int32_t t) {
+  int32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (scenes[mid].tri_begin <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// PerPrimitive.create (renderer/pipeline.py:76-113) for face row t: clip
+// coordinates of its three slots, the [x, y, w] matrix, its determinant,
+// keep / front-facing flags and the inverse.
+struct TriSetup {
+  float clip[3][4];
+  float M[9];
+  float Mi[9];
+  float det;
+  bool keep, front, sane;
+};
+
+JR_DEV void tri_setup(const jr_inputs& in, const jr_scene& sc, int32_t t, TriSetup& o) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int32_t slot = in.faces[3 * (int64_t)t + k];
+    if (in.clip) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o.clip[k][q] = in.clip[4 * (int64_t)slot + q];
+    } else {
+      position_clip(in, sc.world_to_clip, slot, o.clip[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o.M[3 * k + 0] = o.clip[k][0];
+    o.M[3 * k + 1] = o.clip[k][1];
+    o.M[3 * k + 2] = o.clip[k][3];
+  }
+  o.det = det3(o.M);
+  o.keep = fabsf(o.det) > 1e-6f;
+  o.front = o.det >= 0.0f;
+  inv3(o.M, o.Mi);
+  bool sane = true;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) sane = sane && (fabsf(o.Mi[i]) < 1e18f);
+  o.sane = sane;
+}
+
+JR_DEV float4 rec_col(const TriSetup& o, int k) {
+  return make_float4(o.Mi[k], o.Mi[3 + k], o.Mi[6 + k], o.clip[k][2]);
+}
+
+}  // namespace jr

@@ -31,81 +31,22 @@
 #include <vector>
 
 #include "../../include/jrhip.h"
+#include "jr_common.h"
 #include "jr_math.h"
 #include "jr_shade.h"
 
 namespace jr {
 
-// ---------------------------------------------------------------------------
-// triangle records
-//
-// rec[4t+k] for k < 3 is column k of M^-1 with the clip-space z of vertex k:
-// (Minv[0][k], Minv[1][k], Minv[2][k], z_k), so the edge value is
-// e_k = (x * r.x + y * r.y) + r.z.  rec[4t+3] = (det, flags, 0, 0).
-// ---------------------------------------------------------------------------
-constexpr uint32_t kKeep = 1u;     // |det| > 1e-6
-constexpr uint32_t kFront = 2u;    // det >= 0
-constexpr uint32_t kSane = 4u;     // all coefficients finite and < 1e18
-
-struct Rect {
-  int32_t tx0, tx1, ty0, ty1;  // inclusive tile range, empty when tx0 > tx1
-};
-
-// NDC sample position of pixel (i, j) (renderer/pipeline.py:177).
-JR_DEV float pix_x(int32_t i, const float* vp) {
-  return ((float)i - vp[3]) / vp[0];
-}
-JR_DEV float pix_y(int32_t j, const float* vp) {
-  return ((float)j - vp[7]) / vp[5];
-}
-
-// Edge value e_k of the reference: dot((x, y, 1), Minv[:, k]).
-JR_DEV float edge(const float4& r, float x, float y) {
-  float e = x * r.x;
-  e = e + y * r.y;
-  e = e + r.z;
-  return e;
-}
-
-// Window-space depth: interpolate(z_clip, e) * vp[2][2] + vp[2][3]
-// (renderer/pipeline.py:212-219).
-JR_DEV float win_depth(const float4& r0, const float4& r1, const float4& r2,
-                       float e0, float e1, float e2, float vp22, float vp23) {
-  float z = e0 * r0.w;
-  z = z + e1 * r1.w;
-  z = z + e2 * r2.w;
-  return z * vp22 + vp23;
-}
-
-struct Frame {
-  int32_t W, H, TW, TH, tiles_x, tiles_y, T;
-  int32_t n_scenes;
-  int64_t capacity;
-};
-
-JR_DEV int32_t scene_of_tri(const jr_scene* __restrict__ scenes, int32_t n,
-                            int32_t t) {
-  int32_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (scenes[mid].tri_begin <= t)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-// Conservative tile range of a triangle.  The pixel test accepts (x, y) only
+// Conservative pixel range of a triangle.  The pixel test accepts (x, y) only
 // when every float edge value is >= 0; float rounding keeps each computed edge
 // within 3u(|x a| + |y b| + |c|) of its exact value (u = 2^-24), so every
 // accepted pixel lies in the region {a x + b y + c >= -delta_k} with
 // delta_k = 8u(X|a| + Y|b| + |c|).  That region, intersected with the
 // viewport's sample rectangle, is clipped in double precision and its bounding
-// box, widened by one pixel, gives the tile range.
-JR_DEV Rect tile_range(const float4& r0, const float4& r1, const float4& r2,
-                       bool sane, const float* vp, const Frame& f) {
-  Rect full = {0, f.tiles_x - 1, 0, f.tiles_y - 1};
+// box, widened by one pixel, gives the range (inclusive pixel indices).
+JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
+                        bool sane, const float* vp, const Frame& f) {
+  Rect full = {0, f.W - 1, 0, f.H - 1};
   if (!sane || !(vp[0] > 0.0f) || !(vp[5] > 0.0f)) return full;
   const double X0 = pix_x(0, vp), X1 = pix_x(f.W - 1, vp);
   const double Y0 = pix_y(0, vp), Y1 = pix_y(f.H - 1, vp);
@@ -162,93 +103,189 @@ JR_DEV Rect tile_range(const float4& r0, const float4& r1, const float4& r2,
   i1 = fmin(i1, (double)(f.W - 1));
   j1 = fmin(j1, (double)(f.H - 1));
   if (!(i0 <= i1) || !(j0 <= j1)) return r;
-  r.tx0 = (int32_t)i0 / f.TW;
-  r.tx1 = (int32_t)i1 / f.TW;
-  r.ty0 = (int32_t)j0 / f.TH;
-  r.ty1 = (int32_t)j1 / f.TH;
+  r.tx0 = (int32_t)i0;
+  r.tx1 = (int32_t)i1;
+  r.ty0 = (int32_t)j0;
+  r.ty1 = (int32_t)j1;
   return r;
 }
 
-// Exact tile rejection: every computed edge value is monotone in i and in j
-// (IEEE rounding is monotone), so its maximum over a tile sits at a corner
-// pixel.  A tile whose four corners all give e_k < 0 for some k holds no pixel
-// that passes the reference's inside test.
+JR_DEV Rect tiles_of(const Rect& p, const Frame& f) {
+  if (p.tx0 > p.tx1) return p;
+  Rect t = {p.tx0 / f.TW, p.tx1 / f.TW, p.ty0 / f.TH, p.ty1 / f.TH};
+  return t;
+}
+
+// Triangles whose pixel range holds at most this many pixels skip the tile
+// bins: their setup thread evaluates the per-pixel test itself and folds the
+// result into the per-pixel key buffer with a 64-bit atomic minimum.
+constexpr int32_t kSmallPixels = 64;
+
+// (depth key, primitive id) packed so that the unsigned minimum is the
+// chooser's argmin with first-index ties.
+JR_DEV uint64_t pack_key(uint32_t key, int32_t id) {
+  return ((uint64_t)key << 32) | (uint32_t)id;
+}
+
+// Exact tile rejection.  With finite coefficients every computed edge value
+// is monotone in i and in j (IEEE rounding is monotone), so its maximum over a
+// tile sits at a corner pixel.  With an infinite coefficient a (no NaN), x*a
+// only depends on the sign class of x (-inf, NaN at x == 0, +inf) and the
+// finite terms beside it no longer matter, so one pixel per sign class
+// decides.  Testing the pixels {i0, i1, i with x == 0} x {j0, j1, j with
+// y == 0} therefore finds a passing value whenever any pixel of the tile has
+// one; a tile where some edge fails at all of them holds no pixel that passes
+// the reference's inside test.
+JR_DEV int32_t zero_pixel(float v3, int32_t lo, int32_t hi) {
+  // the pixel whose NDC coordinate is exactly 0, if it lies in [lo, hi]
+  const float r = floorf(v3);
+  if (r != v3 || !(r >= (float)lo) || !(r <= (float)hi)) return -1;
+  return (int32_t)r;
+}
+
 JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
                            bool sane, const float* vp, const Frame& f,
                            int32_t tx, int32_t ty) {
-  if (!sane) return true;
   const int32_t i0 = tx * f.TW, i1 = min(i0 + f.TW, f.W) - 1;
   const int32_t j0 = ty * f.TH, j1 = min(j0 + f.TH, f.H) - 1;
-  const float xa = pix_x(i0, vp), xb = pix_x(i1, vp);
-  const float ya = pix_y(j0, vp), yb = pix_y(j1, vp);
+  float xs[3] = {pix_x(i0, vp), pix_x(i1, vp), 0.0f};
+  float ys[3] = {pix_y(j0, vp), pix_y(j1, vp), 0.0f};
+  int nx = 2, ny = 2;
+  if (!sane) {
+    if (!(vp[0] > 0.0f) || !(vp[5] > 0.0f)) return true;
+    const int32_t iz = zero_pixel(vp[3], i0, i1), jz = zero_pixel(vp[7], j0, j1);
+    if (iz >= 0) xs[nx++] = pix_x(iz, vp);
+    if (jz >= 0) ys[ny++] = pix_y(jz, vp);
+  }
   const float4 rs[3] = {r0, r1, r2};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const float e00 = edge(rs[k], xa, ya), e01 = edge(rs[k], xa, yb);
-    const float e10 = edge(rs[k], xb, ya), e11 = edge(rs[k], xb, yb);
-    if (e00 < 0.0f && e01 < 0.0f && e10 < 0.0f && e11 < 0.0f) return false;
+    bool pass = false;
+    for (int p = 0; p < nx; ++p)
+      for (int q = 0; q < ny; ++q) pass = pass || (edge(rs[k], xs[p], ys[q]) >= 0.0f);
+    if (!pass) return false;
   }
   return true;
 }
 
-__global__ void k_zero(int32_t* __restrict__ p, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0;
+// Triangles whose tile range spans more than this many tiles are binned by a
+// whole wavefront (one lane per tile) instead of by their own setup thread.
+constexpr int32_t kWideTiles = 16;
+
+JR_DEV bool has_nan_column(const float4& r) {
+  return (r.x != r.x) || (r.y != r.y) || (r.z != r.z);
 }
 
-// Stage 1: vertex positions -> clip space, PerPrimitive.create, tile counts.
+// Clears the tile counters (n32 int32) and sets the per-pixel keys (n64
+// uint64) to "no candidate".
+__global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __restrict__ p64,
+                        int64_t n64) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n32; i += stride)
+    p32[i] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n64; i += stride)
+    p64[i] = ~0ull;
+}
+
+// Stage 1: vertex positions -> clip space, PerPrimitive.create; small
+// triangles are resolved into the key buffer here, large ones counted into
+// the tiles they may cover.
 __global__ __launch_bounds__(256) void k_setup(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     float4* __restrict__ rec, Rect* __restrict__ rects,
-    int32_t* __restrict__ counts) {
+    int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
+    int32_t* __restrict__ wide, int32_t* __restrict__ wide_count) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= in.n_tris) return;
   const int32_t s = scene_of_tri(scenes, f.n_scenes, t);
   const jr_scene& sc = scenes[s];
-  float clip[3][4];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int32_t slot = in.faces[3 * (int64_t)t + k];
-    if (in.clip) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) clip[k][q] = in.clip[4 * (int64_t)slot + q];
-    } else {
-      position_clip(in, sc.world_to_clip, slot, clip[k]);
-    }
-  }
-  // matrix with rows (x, y, w) of the three vertices
-  float M[9];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    M[3 * k + 0] = clip[k][0];
-    M[3 * k + 1] = clip[k][1];
-    M[3 * k + 2] = clip[k][3];
-  }
-  const float det = det3(M);
-  const bool keep = fabsf(det) > 1e-6f;
-  const bool front = det >= 0.0f;
-  float Mi[9];
-  inv3(M, Mi);
-  bool sane = true;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) sane = sane && (fabsf(Mi[i]) < 1e18f);
+  TriSetup ts;
+  tri_setup(in, sc, t, ts);
+  const bool keep = ts.keep, front = ts.front, sane = ts.sane;
+  const float det = ts.det;
   uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
-  const float4 r0 = make_float4(Mi[0], Mi[3], Mi[6], clip[0][2]);
-  const float4 r1 = make_float4(Mi[1], Mi[4], Mi[7], clip[1][2]);
-  const float4 r2 = make_float4(Mi[2], Mi[5], Mi[8], clip[2][2]);
+  const float4 r0 = rec_col(ts, 0);
+  const float4 r1 = rec_col(ts, 1);
+  const float4 r2 = rec_col(ts, 2);
   rec[4 * (int64_t)t + 0] = r0;
   rec[4 * (int64_t)t + 1] = r1;
   rec[4 * (int64_t)t + 2] = r2;
   rec[4 * (int64_t)t + 3] = make_float4(det, __uint_as_float(flags), 0.0f, 0.0f);
   Rect r = {1, 0, 1, 0};
-  if (keep && front) r = tile_range(r0, r1, r2, sane, sc.viewport, f);
+  Rect px = r;
+  // a NaN coefficient makes that edge value NaN at every pixel: never inside
+  const bool nan_col = has_nan_column(r0) || has_nan_column(r1) || has_nan_column(r2);
+  if (keep && front && !nan_col) px = pixel_range(r0, r1, r2, sane, sc.viewport, f);
+  const int64_t area = (px.tx0 > px.tx1)
+      ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
+  if (area > kSmallPixels) r = tiles_of(px, f);
   rects[t] = r;
+  if (area > 0 && area <= kSmallPixels) {
+    // the tile raster's exact per-pixel arithmetic, pixel by pixel
+    const float* vp = sc.viewport;
+    const float vp22 = vp[10], vp23 = vp[11];
+    uint64_t* kb = keys + (int64_t)s * f.W * f.H;
+    for (int32_t i = px.tx0; i <= px.tx1; ++i) {
+      const float x = pix_x(i, vp);
+      for (int32_t j = px.ty0; j <= px.ty1; ++j) {
+        const float y = pix_y(j, vp);
+        const float e0 = edge(r0, x, y), e1 = edge(r1, x, y), e2 = edge(r2, x, y);
+        if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
+          const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
+          atomicMin((unsigned long long*)(kb + (int64_t)i * f.H + j),
+                    (unsigned long long)pack_key(key, t));
+        }
+      }
+    }
+  }
+  const int64_t n_t = (r.tx0 > r.tx1) ? 0 : (int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
+  if (n_t > kWideTiles) {
+    wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
+    return;
+  }
   int32_t* cnt = counts + (int64_t)s * f.T;
   for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
     for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
       if (tile_may_cover(r0, r1, r2, sane, sc.viewport, f, tx, ty))
         atomicAdd(cnt + tx * f.tiles_y + ty, 1);
+}
+
+// Counts (BIN = false) or bins (BIN = true) the wide triangles: one wavefront
+// per triangle, its lanes striding over the triangle's tiles.
+template <bool BIN>
+__global__ __launch_bounds__(256) void k_wide(
+    const jr_scene* __restrict__ scenes, Frame f, const float4* __restrict__ rec,
+    const Rect* __restrict__ rects, const int32_t* __restrict__ wide,
+    const int32_t* __restrict__ wide_count, int32_t* __restrict__ counts,
+    const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
+    int32_t* __restrict__ bins) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int32_t n = *wide_count;
+  for (int64_t w = wave; w < n; w += n_waves) {
+    const int32_t t = wide[w];
+    const Rect r = rects[t];
+    const int32_t s = scene_of_tri(scenes, f.n_scenes, t);
+    const float* vp = scenes[s].viewport;
+    const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
+                 r2 = rec[4 * (int64_t)t + 2];
+    const bool sane = (__float_as_uint(rec[4 * (int64_t)t + 3].y) & kSane) != 0;
+    const int32_t h = r.ty1 - r.ty0 + 1;
+    const int32_t n_t = (r.tx1 - r.tx0 + 1) * h;
+    const int64_t tbase = (int64_t)s * f.T;
+    for (int32_t q = lane; q < n_t; q += 64) {
+      const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
+      if (!tile_may_cover(r0, r1, r2, sane, vp, f, tx, ty)) continue;
+      const int64_t gt = tbase + tx * f.tiles_y + ty;
+      if (BIN) {
+        const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
+        if (pos < f.capacity) bins[pos] = t;
+      } else {
+        atomicAdd(counts + gt, 1);
+      }
+    }
+  }
 }
 
 // Stage 2: exclusive scan of n counts -> offsets (two levels, 2048 per block).
@@ -330,6 +367,7 @@ __global__ __launch_bounds__(256) void k_bin(
   if (t >= in.n_tris) return;
   const Rect r = rects[t];
   if (r.tx0 > r.tx1) return;
+  if ((int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1) > kWideTiles) return;  // k_wide
   const int32_t s = scene_of_tri(scenes, f.n_scenes, t);
   const float* vp = scenes[s].viewport;
   const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
@@ -355,7 +393,9 @@ struct RasterArgs {
   const int32_t* counts;
   const int32_t* offsets;
   const int32_t* bins;
+  const uint64_t* keys;
   int32_t* work_counter;
+  int32_t chunk;  // tiles claimed per work-queue step (persistent form)
   int32_t init_buffers, apply_offset;
   float zdef, cdef0, cdef1, cdef2, depth_offset;
 };
@@ -383,8 +423,16 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
   // overflowing tile lists fall back to every triangle of the scene
   const int32_t n_items = overflow ? (sc.tri_end - sc.tri_begin) : cnt;
 
+  // start from the small triangles already resolved by k_setup
   uint32_t best_key = 0xFFFFFFFFu;
   int32_t best_id = 0x7FFFFFFF;
+  if (inside) {
+    const uint64_t k0 = a.keys[(int64_t)s * f.W * f.H + (int64_t)i * f.H + j];
+    if (k0 != ~0ull) {
+      best_key = (uint32_t)(k0 >> 32);
+      best_id = (int32_t)(uint32_t)k0;
+    }
+  }
   for (int32_t base = 0; base < n_items; base += NT) {
     const int32_t n = min(NT, n_items - base);
     __syncthreads();
@@ -503,11 +551,12 @@ __global__ __launch_bounds__(TW* TH) void k_raster_persistent(RasterArgs a,
   __shared__ int32_t item;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) item = atomicAdd(a.work_counter, 1);
+    if (threadIdx.x == 0) item = atomicAdd(a.work_counter, a.chunk);
     __syncthreads();
-    const int32_t gt = item;
-    if (gt >= n_items) break;
-    raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
+    const int32_t first = item;
+    if (first >= n_items) break;
+    const int32_t last = (int32_t)min((int64_t)first + a.chunk, n_items);
+    for (int32_t gt = first; gt < last; ++gt) raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
   }
 }
 
@@ -554,8 +603,8 @@ constexpr int64_t kAlign = 256;
 int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
-  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, bins, total;
-  int64_t n_tiles, nblk;
+  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, bins, total;
+  int64_t n_tiles, nblk, n_pix;
 };
 
 WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
@@ -569,9 +618,12 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.rects = o;      o += align_up(16 * (int64_t)std::max(n_tris, 1));
   L.counts = o;     o += align_up(4 * L.n_tiles);
   L.cursor = o;     o += align_up(4 * L.n_tiles);
-  L.misc = o;       o += align_up(64);   // [0] total pairs, [1] work counter
+  L.misc = o;       o += align_up(64);   // [0] total pairs, [1] work counter, [2] wide count
   L.offsets = o;    o += align_up(4 * (L.n_tiles + 1));
   L.block_sums = o; o += align_up(4 * std::max<int64_t>(L.nblk, 1));
+  L.n_pix = (int64_t)fr->width * fr->height * fr->n_scenes;
+  L.keys = o;       o += align_up(8 * L.n_pix);
+  L.wide = o;       o += align_up(4 * (int64_t)std::max(n_tris, 1));
   L.bins = o;       o += align_up(4 * std::max<int64_t>(fr->bin_capacity, 1));
   L.total = o;
   return L;
@@ -602,8 +654,12 @@ void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int per_cu = 2048 / (TW * TH);  // up to 32 waves per CU
     const int64_t grid = std::min<int64_t>(n_items, (int64_t)std::max(cus, 1) * per_cu);
+    // claim tiles in chunks: one queue atomic per chunk, about eight chunks
+    // per group
+    RasterArgs b = a;
+    b.chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(32, n_items / (grid * 8)));
     hipLaunchKernelGGL((k_raster_persistent<SHADER, TW, TH>), dim3((unsigned)grid),
-                       dim3(TW * TH), 0, st, a, n_items);
+                       dim3(TW * TH), 0, st, b, n_items);
   } else {
     hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)n_items),
                        dim3(TW * TH), 0, st, a);
@@ -721,17 +777,25 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   int32_t* offsets = reinterpret_cast<int32_t*>(ws + L.offsets);
   int32_t* block_sums = reinterpret_cast<int32_t*>(ws + L.block_sums);
   int32_t* bins = reinterpret_cast<int32_t*>(ws + L.bins);
+  uint64_t* keys = reinterpret_cast<uint64_t*>(ws + L.keys);
+  int32_t* wide = reinterpret_cast<int32_t*>(ws + L.wide);
+  int32_t* wide_count = misc + 2;
+  const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
   StageTimer timer(st);
-  // counts, cursor and misc are contiguous: clear them with one fill kernel
-  // (a kernel rather than hipMemsetAsync so the sequence can be graph-captured)
+  // counts, cursor and misc are contiguous: clear them, and the key buffer,
+  // with one fill kernel (a kernel rather than hipMemsetAsync so the sequence
+  // can be graph-captured)
   const int64_t nz = (L.offsets - L.counts) / 4;
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((nz + 255) / 256, 4096)),
-                     dim3(256), 0, st, counts, nz);
+  const int64_t nclr = std::max(nz, L.n_pix);
+  hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<int64_t>((nclr + 255) / 256, 8192)),
+                     dim3(256), 0, st, counts, nz, keys, L.n_pix);
   const int n_tris = in->n_tris;
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_setup, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rec, rects, counts);
+                       scenes, f, rec, rects, counts, keys, wide, wide_count);
+    hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
+                       rects, wide, wide_count, counts, offsets, cursor, bins);
   }
   timer.mark(1);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
@@ -744,6 +808,8 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
                        scenes, f, rec, rects, offsets, cursor, bins);
+    hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
+                       rects, wide, wide_count, counts, offsets, cursor, bins);
   }
   timer.mark(3);
   RasterArgs a;
@@ -755,7 +821,9 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.counts = counts;
   a.offsets = offsets;
   a.bins = bins;
+  a.keys = keys;
   a.work_counter = misc + 1;
+  a.chunk = 1;
   a.init_buffers = fr->init_buffers;
   a.apply_offset = fr->apply_offset;
   a.zdef = fr->zbuffer_default;

@@ -175,3 +175,64 @@ def soup_call(shader: type, seed: int, width: int = 64, height: int = 48, n_tris
             main.shadow_from = "shadow"
             return [shadow_call, main]
     return [build_call(camera, shader, faces, ex, width, height, **kw)]
+
+
+def soup_inputs(shader: type, seed: int, width: int = 48, height: int = 40, n_tris: int = 40):
+    """Raw inputs of a seeded soup for the gradient tests: (camera, faces,
+    extra, shadow_call) with numpy arrays; shadow_call is the depth pass whose
+    zbuffer feeds extra.shadow.shadow_map (None for other shaders)."""
+    calls = soup_call(shader, seed, width, height, n_tris)
+    rng = np.random.default_rng(seed)
+    V = 3 * n_tris
+    pos = _soup(rng, n_tris)
+    faces = rng.permutation(V).astype(np.int32).reshape(n_tris, 3)
+    normal = rng.normal(size=(V, 3)).astype(f32)
+    uv = rng.uniform(-2.0, 3.0, size=(V, 2)).astype(f32)
+    colour = rng.uniform(-0.2, 1.0, size=(V, 3)).astype(f32)
+    tex = rng.uniform(0.0, 1.0, size=(7, 5, 3)).astype(f32)
+    camera = Camera.create(
+        view=Camera.view_matrix(eye=np.array((0.3, 0.2, 3.0), f32), centre=np.zeros(3, f32),
+                                up=np.array((0.0, 1.0, 0.0), f32)),
+        projection=Camera.perspective_projection_matrix(60.0, width / height, 0.1, 20.0),
+        viewport=Camera.viewport_matrix(np.zeros(2, np.int32), np.array((width, height)), 1.0),
+    )
+    light = LightSource(direction=np.array((0.3, 0.5, 0.8), f32),
+                        colour=np.array((1.0, 0.9, 0.8), f32))
+    shadow_call = calls[0] if len(calls) == 2 else None
+    if shader is DepthShader:
+        return camera, faces, DepthExtraInput(position=pos), None
+    if shader is GouraudShader:
+        return camera, faces, GouraudExtraInput(position=pos, colour=colour, normal=normal,
+                                                light=light), None
+    if shader is GouraudTextureShader:
+        return camera, faces, GouraudTextureExtraInput(position=pos, normal=normal, uv=uv,
+                                                       light=light, texture=tex), None
+    if shader is PhongTextureShader:
+        return camera, faces, PhongTextureExtraInput(position=pos, normal=normal, uv=uv,
+                                                     light=light, texture=tex), None
+    main = calls[-1]
+    # the reflection shaders' soup draws its extra arrays after the ones above
+    n_obj = 3
+    texm = rng.uniform(0.0, 1.0, size=(4 * n_obj, 6, 3)).astype(f32)
+    spec = rng.uniform(0.0, 4.0, size=(4 * n_obj, 6)).astype(f32)
+    tshape = np.array([[4, 6], [3, 5], [4, 2]], np.int32)
+    tindex = rng.integers(0, n_obj, size=V).astype(np.int32)
+    ldir = np.array((0.3, 0.5, 0.8), f32)
+    common = dict(position=pos, normal=normal, uv=uv, light=light,
+                  light_dir_eye=Camera.apply_vec(ldir, camera.view), texture_shape=tshape,
+                  texture_index=tindex, texture_offset=4, texture=texm, specular_map=spec,
+                  ambient=np.array((0.3, 0.3, 0.3), f32), diffuse=np.array((0.5, 0.6, 0.7), f32),
+                  specular=np.array((0.4, 0.3, 0.2), f32))
+    assert np.array_equal(main.inputs["textures"], texm)
+    if shader is PhongReflectionTextureShader:
+        return camera, faces, PhongReflectionTextureExtraInput(**common), None
+    scam = None
+    SW, SH = shadow_call.width, shadow_call.height
+    vp = Camera.viewport_matrix(np.zeros(2, np.int32), np.array((SW, SH)), 1.0)
+    scam = Shadow.camera_for(np.array((0.2, 0.3, 1.0), f32), vp, np.zeros(3, f32),
+                             np.array((0.0, 1.0, 0.0), f32), distance=4.0)
+    scam = scam._replace(world_to_clip=(scam.world_to_clip * np.array(
+        [[0.5], [0.5], [0.2], [1.0]], f32)).astype(f32))
+    sh = Shadow(shadow_map=None, strength=np.array((0.6, 0.5, 0.4), f32), camera=scam)
+    return camera, faces, PhongReflectionShadowTextureExtraInput(**common, shadow=sh,
+                                                                 camera=camera), shadow_call

@@ -64,3 +64,20 @@ def test_merge_into_existing_buffers():
 
 def test_reference_smoke_scene_small():
     _check(lambda: [scenes.smoke_gouraud_call(192, 108)], "smoke192")
+
+
+def test_dense_sphere_small_triangles():
+    """Config-2 sphere (9.2k triangles, many smaller than a pixel) at 240x135:
+    exercises the per-pixel atomic path of small triangles."""
+    from jaxrenderer_amd import batch, workloads
+
+    sc = workloads.config2(seed=1, width=240, height=135)
+    _check(lambda: batch.plan_batch([sc], 240, 135), "sphere240")
+
+
+def test_config3_batch_against_oracle():
+    """Three uneven Brax-like scenes in one batched call, 256x256 shadow maps."""
+    from jaxrenderer_amd import batch, workloads
+
+    scs, sm = workloads.config3(n_scenes=3, seed=11)
+    _check(lambda: batch.plan_batch(scs, 84, 84, sm), "c3x3")
