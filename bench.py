@@ -47,12 +47,17 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tile", type=int, nargs=2, default=(8, 8))
-    ap.add_argument("--shadow-tile", type=int, nargs=2, default=(8, 8))
+    ap.add_argument("--tile", default="auto",
+                    help="'auto' (time every launch shape on this batch) or WxH, e.g. 8x8")
+    ap.add_argument("--shadow-tile", default=None, help="WxH for the shadow pass (with --tile WxH)")
+    ap.add_argument("--no-autotune-cache", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-scene", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy, the reference "
+                         "kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build renders on one GPU; --gpus must be 1 "
@@ -64,6 +69,14 @@ def main() -> None:
 
     from jaxrenderer_amd import _hip, batch, workloads
 
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+        del src, dst
+
     W = H = 84
     B = args.scenes
     t0 = time.time()
@@ -73,8 +86,22 @@ def main() -> None:
     log(f"generated {B} scenes ({tris.sum()} triangles, {tris.min()}..{tris.max()} per scene) "
         f"in {time.time() - t0:.1f}s")
 
-    br = batch.BatchRenderer(calls, tile=tuple(args.tile), shadow_tile=tuple(args.shadow_tile),
-                             persistent=True)
+    def wh(s):
+        w, h = (int(v) for v in s.lower().split("x"))
+        return (w, h)
+
+    auto = args.tile == "auto"
+    tile = (8, 8) if auto else wh(args.tile)
+    shadow_tile = wh(args.shadow_tile) if args.shadow_tile else tile
+    br = batch.BatchRenderer(calls, tile=tile, shadow_tile=shadow_tile, persistent=True)
+    tuned = None
+    if auto:
+        t0 = time.time()
+        tuned = br.autotune(use_cache=not args.no_autotune_cache)
+        for tag, c in tuned.items():
+            log(f"autotune {tag}: tile {c['tile']} persistent={c['persistent']} "
+                f"{c['ms']:.3f} ms ({'cached' if c['cached'] else 'timed'})")
+        log(f"autotune took {time.time() - t0:.1f}s")
     pairs = br.calibrate()
     log(f"binned triangle-tile pairs: {pairs}")
     br.capture()
@@ -142,8 +169,7 @@ def main() -> None:
         singles = []
         for i in range(B):
             r = batch.BatchRenderer(batch.plan_batch([scenes[i]], W, H, smspec),
-                                    tile=tuple(args.tile), shadow_tile=tuple(args.shadow_tile),
-                                    persistent=False)
+                                    tile=(8, 8), shadow_tile=(8, 8), persistent=False)
             singles.append(r)
         for r in singles[: min(8, B)]:
             r.render()
@@ -231,7 +257,9 @@ def main() -> None:
             "width": W,
             "height": H,
             "shadow_map": [smspec.width, smspec.height],
-            "tile": list(args.tile),
+            "tile": {tag: {"tile": list(p.tile), "persistent": p.persistent}
+                     for tag, p in br.passes.items()},
+            "autotuned": auto,
         },
         "mpix_per_s": round(fps * W * H / 1e6, 2),
         "per_scene_fps": None if per_scene_fps is None else round(per_scene_fps, 1),

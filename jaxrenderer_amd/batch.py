@@ -239,6 +239,13 @@ class BatchRenderer:
         self.graph = None
         return got
 
+    def autotune(self, **kw) -> Dict[str, dict]:
+        """Chooses tile shape, launch form and bin capacity per pass by timing
+        them on this batch (jaxrenderer_amd.autotune.tune)."""
+        from . import autotune
+
+        return autotune.tune(self, **kw)
+
     def capture(self) -> None:
         """Captures render() into a HIP graph (torch.cuda.CUDAGraph)."""
         self.render()

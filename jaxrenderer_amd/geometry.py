@@ -40,8 +40,26 @@ def _sum_sq(v: np.ndarray) -> np.float32:
     return f32(np.sum(flat * flat, dtype=f32))
 
 
-def normalise(vector: ArrayLike) -> np.ndarray:
-    """renderer/geometry.py:36-47: divide by the 2-norm of the WHOLE array."""
+def _is_torch(x) -> bool:
+    return hasattr(x, "detach") and hasattr(x, "device")
+
+
+def _torch_matrix(m, like):
+    import torch
+
+    if _is_torch(m):
+        return m.to(device=like.device, dtype=like.dtype)
+    return torch.as_tensor(np.asarray(m), device=like.device, dtype=like.dtype)
+
+
+def normalise(vector: ArrayLike):
+    """renderer/geometry.py:36-47: divide by the 2-norm of the WHOLE array.
+    Torch tensors stay torch (and differentiable); anything else becomes a
+    float32 numpy array."""
+    if _is_torch(vector):
+        import torch
+
+        return vector / torch.linalg.norm(vector)
     v = asf32(vector)
     n = f32(np.sqrt(_sum_sq(v)))
     return (v / n).astype(f32)
@@ -146,19 +164,31 @@ def interpolate(values, barycentric_screen, barycentric_clip,
     return mode(values, barycentric_screen, barycentric_clip)
 
 
-def to_homogeneous(coordinates: ArrayLike, value: float = 1.0) -> np.ndarray:
+def to_homogeneous(coordinates: ArrayLike, value: float = 1.0):
+    if _is_torch(coordinates):
+        import torch
+
+        pad = torch.full_like(coordinates[..., :1], float(value))
+        return torch.cat([coordinates, pad], dim=-1)
     c = asf32(coordinates)
     pad = np.full(c.shape[:-1] + (1,), value, dtype=f32)
     return np.concatenate([c, pad], axis=-1)
 
 
-def normalise_homogeneous(coordinates: ArrayLike) -> np.ndarray:
+def normalise_homogeneous(coordinates: ArrayLike):
+    if _is_torch(coordinates):
+        return coordinates / coordinates[..., -1:]
     c = asf32(coordinates)
     with np.errstate(all="ignore"):
         return (c / c[..., -1:]).astype(f32)
 
 
-def to_cartesian(coordinates: ArrayLike) -> np.ndarray:
+def to_cartesian(coordinates: ArrayLike):
+    if _is_torch(coordinates):
+        import torch
+
+        return torch.where(coordinates[..., -1:] == 0.0, coordinates[..., :-1],
+                           normalise_homogeneous(coordinates)[..., :-1])
     c = asf32(coordinates)
     with np.errstate(all="ignore"):
         return np.where(c[..., -1:] == 0.0, c[..., :-1],
@@ -206,6 +236,9 @@ class Camera(NamedTuple):
     @staticmethod
     def apply(points: ArrayLike, matrix: ArrayLike) -> np.ndarray:
         """out_i = sum_j p_j M[i, j] (renderer/geometry.py:280-315)."""
+        if _is_torch(points):
+            M = _torch_matrix(matrix, points)
+            return points @ M.transpose(0, 1)
         p = asf32(points)
         M = asf32(matrix)
         single = p.ndim == 1

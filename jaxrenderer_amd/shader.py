@@ -65,9 +65,8 @@ class Shader:
         (renderer/shader.py:207-251).  Operates on [primitives, ...] tensors."""
         inf = torch.full_like(gl_FragCoord[:, 2], float("inf"))
         depths = torch.where(keeps & gl_FrontFacing, gl_FragCoord[:, 2], inf)
-        idx = torch.argmin(depths)
-        sl = slice(None)
-        take = lambda a: a[idx][None, ...]  # noqa: E731
+        idx = torch.argmin(depths).reshape(1)
+        take = lambda a: torch.index_select(a, 0, idx)  # noqa: E731
         return (
             take(gl_FragCoord),
             take(gl_FrontFacing),
@@ -81,10 +80,9 @@ class Shader:
     @staticmethod
     def interpolate(values, barycentric_screen, barycentric_clip):
         """Default: every field SMOOTH (perspective-correct) interpolated."""
-        return type(values)(
-            *(interpolate(v, barycentric_screen, barycentric_clip, Interpolation.SMOOTH)
-              for v in values)
-        )
+        out = tuple(interpolate(v, barycentric_screen, barycentric_clip, Interpolation.SMOOTH)
+                    for v in values)
+        return type(values)(*out) if hasattr(values, "_fields") else out
 
     @staticmethod
     def fragment(gl_FragCoord, gl_FrontFacing, gl_PointCoord, varying, extra):
@@ -96,10 +94,11 @@ class Shader:
         """Default: the kept fragment of smallest depth (renderer/shader.py:374-396)."""
         inf = torch.full_like(gl_FragDepth, float("inf"))
         depths = torch.where(keeps, gl_FragDepth, inf)
-        idx = torch.argmin(depths)
-        return MixerOutput(keep=keeps[idx], zbuffer=depths[idx]), type(extra)(
-            *(v[idx] for v in extra)
-        )
+        idx = torch.argmin(depths).reshape(1)
+        pick = lambda a: torch.index_select(a, 0, idx).squeeze(0)  # noqa: E731
+        picked = tuple(pick(v) for v in extra)
+        return MixerOutput(keep=pick(keeps), zbuffer=pick(depths)), (
+            type(extra)(*picked) if hasattr(extra, "_fields") else picked)
 
 
 __all__ = ["ID", "MixerOutput", "PerFragment", "PerVertex", "Shader"]
