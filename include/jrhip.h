@@ -128,6 +128,12 @@ typedef struct jr_outputs {
   float* canvas;    /* NULL for DEPTH / VISIBILITY */
   int32_t* prim_id; /* optional: chosen face row where written, else -1 */
   float* bary;      /* optional: perspective-correct barycentrics (3) */
+  /* optional: [n_scenes][height][width][3] bytes, the display image of the
+   * merged canvas: transpose_for_display((clamp(0, canvas, 1) * 255)
+   * .astype(uint8)) (renderer/utils.py:72-96, the output step of
+   * notebooks/84x84/A100.ipynb cell 7): axes swapped, rows flipped, values
+   * truncated toward zero, NaN -> 0.  Written for every pixel. */
+  uint8_t* display;
 } jr_outputs;
 
 typedef struct jr_frame {

@@ -100,6 +100,7 @@ class JrOutputs(ctypes.Structure):
         ("canvas", c_vp),
         ("prim_id", c_vp),
         ("bary", c_vp),
+        ("display", c_vp),
     ]
 
 

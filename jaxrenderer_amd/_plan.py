@@ -61,7 +61,7 @@ def to_device(x, dev: torch.device, name: str) -> Optional[torch.Tensor]:
 
 def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         canvas: Optional[torch.Tensor] = None, prim_id: bool = False, bary: bool = False,
-        tile=_raster.DEFAULT_TILE, persistent: bool = False,
+        display: bool = False, tile=_raster.DEFAULT_TILE, persistent: bool = False,
         capacity: Optional[int] = None) -> Dict[str, torch.Tensor]:
     """Executes `call` on the GPU.  zbuffer/canvas given here are updated in
     place (merge semantics when call.init_buffers is False)."""
@@ -84,12 +84,16 @@ def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         out["prim_id"] = pid
     if bar is not None:
         out["bary"] = bar
+    disp = (torch.empty((B, H, W, 3), dtype=torch.uint8, device=dev)
+            if display and canvas is not None else None)
+    if disp is not None:
+        out["display"] = disp
     inputs = {k: to_device(v, dev, k) for k, v in call.inputs.items()}
     scenes = _raster.scenes_to_device(call.scenes, dev)
     _raster.forward(
         shader=call.shader, width=W, height=H, scenes=scenes, n_scenes=B, inputs=inputs,
         n_slots=call.n_slots, n_tris=call.n_tris, zbuffer=zbuffer, canvas=canvas,
-        prim_id=pid, bary=bar, shadow_shape=call.shadow_shape,
+        prim_id=pid, bary=bar, display=disp, shadow_shape=call.shadow_shape,
         init_buffers=call.init_buffers, zbuffer_default=call.zbuffer_default,
         colour_default=call.colour_default, apply_offset=call.apply_offset,
         depth_offset=call.depth_offset, tile=tile, persistent=persistent, capacity=capacity,

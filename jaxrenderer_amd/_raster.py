@@ -195,6 +195,7 @@ def forward(
     canvas: Optional[torch.Tensor] = None,
     prim_id: Optional[torch.Tensor] = None,
     bary: Optional[torch.Tensor] = None,
+    display: Optional[torch.Tensor] = None,
     shadow_shape: Tuple[int, int] = (0, 0),
     init_buffers: bool = False,
     zbuffer_default: float = 1.0,
@@ -229,6 +230,9 @@ def forward(
     jout.canvas = _ptr(canvas, "canvas", torch.float32, dev)
     jout.prim_id = _ptr(prim_id, "prim_id", torch.int32, dev)
     jout.bary = _ptr(bary, "bary", torch.float32, dev)
+    jout.display = _ptr(display, "display", torch.uint8, dev)
+    if display is not None and tuple(display.shape) != (n_scenes, height, width, 3):
+        raise ValueError(f"display must be [{n_scenes}, {height}, {width}, 3] uint8")
     if scenes.device != dev or scenes.dtype != torch.uint8:
         raise ValueError("scenes must be a uint8 device tensor of jr_scene structs")
     if scenes.numel() != n_scenes * ctypes.sizeof(_hip.JrScene):

@@ -156,7 +156,7 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
 
 class _Pass:
     def __init__(self, call: RasterCall, dev: torch.device, tile, persistent: bool,
-                 want_ids: bool):
+                 want_ids: bool, want_display: bool = False):
         self.call = call
         self.tile = tile
         self.persistent = persistent
@@ -168,6 +168,8 @@ class _Pass:
                        if call.want_canvas else None)
         self.prim_id = torch.empty((B, W, H), dtype=torch.int32, device=dev) if want_ids else None
         self.bary = torch.empty((B, W, H, 3), dtype=torch.float32, device=dev) if want_ids else None
+        self.display = (torch.empty((B, H, W, 3), dtype=torch.uint8, device=dev)
+                        if want_display and call.want_canvas else None)
         self.set_capacity(_raster.default_capacity(call.n_tris, self.n_tiles))
 
     @property
@@ -188,7 +190,7 @@ class _Pass:
             shader=c.shader, width=c.width, height=c.height, scenes=self.scenes,
             n_scenes=c.n_scenes, inputs=self.inputs, n_slots=c.n_slots, n_tris=c.n_tris,
             zbuffer=self.zbuffer, canvas=self.canvas, prim_id=self.prim_id, bary=self.bary,
-            shadow_shape=c.shadow_shape, init_buffers=c.init_buffers,
+            display=self.display, shadow_shape=c.shadow_shape, init_buffers=c.init_buffers,
             zbuffer_default=c.zbuffer_default, colour_default=c.colour_default,
             apply_offset=c.apply_offset, depth_offset=c.depth_offset, tile=self.tile,
             persistent=self.persistent, capacity=self.capacity, workspace=self.workspace)
@@ -204,12 +206,16 @@ class BatchRenderer:
     """Device-resident batched renderer over a plan from `plan_batch`."""
 
     def __init__(self, calls: List[RasterCall], *, tile=(8, 8), shadow_tile=None,
-                 persistent: bool = True, want_ids: bool = False):
+                 persistent: bool = True, want_ids: bool = False, display: bool = False):
+        """`display`: also write the main pass's uint8 display images
+        [B, H, W, 3] (clamp to [0, 1], x 255, truncate, transpose_for_display)."""
         dev = _raster.require_device()
         self.passes: Dict[str, _Pass] = {}
         for c in calls:
             t = shadow_tile if (c.tag == "shadow" and shadow_tile is not None) else tile
-            self.passes[c.tag] = _Pass(c, dev, t, persistent, want_ids and c.tag == "main")
+            main = c.tag == "main"
+            self.passes[c.tag] = _Pass(c, dev, t, persistent, want_ids and main,
+                                       display and main)
         main = self.passes["main"]
         if main.call.shadow_from is not None:
             main.inputs["shadow_maps"] = self.passes[main.call.shadow_from].zbuffer
@@ -222,6 +228,8 @@ class BatchRenderer:
         out = {"zbuffer": main.zbuffer, "canvas": main.canvas}
         if main.prim_id is not None:
             out.update(prim_id=main.prim_id, bary=main.bary)
+        if main.display is not None:
+            out["display"] = main.display
         if "shadow" in self.passes:
             out["shadow_map"] = self.passes["shadow"].zbuffer
         return out
@@ -263,6 +271,8 @@ class BatchRenderer:
         out = {"zbuffer": main.zbuffer, "canvas": main.canvas}
         if main.prim_id is not None:
             out.update(prim_id=main.prim_id, bary=main.bary)
+        if main.display is not None:
+            out["display"] = main.display
         return out
 
     # ---- reverse mode -----------------------------------------------------

@@ -526,6 +526,25 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
       a.out.zbuffer[pix] = a.out.zbuffer[pix] + a.depth_offset;
     }
   }
+  if (a.out.display) {
+    // the merged canvas value, transposed and flipped for display
+    float c0 = col[0], c1 = col[1], c2 = col[2];
+    if (!keep) {
+      if (a.init_buffers) {
+        c0 = a.cdef0;
+        c1 = a.cdef1;
+        c2 = a.cdef2;
+      } else {
+        c0 = a.out.canvas[3 * pix + 0];
+        c1 = a.out.canvas[3 * pix + 1];
+        c2 = a.out.canvas[3 * pix + 2];
+      }
+    }
+    uint8_t* d = a.out.display + 3 * (((int64_t)s * f.H + (f.H - 1 - j)) * f.W + i);
+    d[0] = display_u8(c0);
+    d[1] = display_u8(c1);
+    d[2] = display_u8(c2);
+  }
   if (a.out.prim_id) a.out.prim_id[pix] = keep ? w : -1;
   if (a.out.bary) {
     a.out.bary[3 * pix + 0] = keep ? b[0] : 0.0f;
@@ -702,6 +721,8 @@ int32_t check_inputs(const jr_frame* fr, const jr_inputs* in, const jr_outputs* 
   const int sh = fr->shader;
   const bool needs_canvas = !(sh == JR_SHADER_DEPTH || sh == JR_SHADER_VISIBILITY);
   if (needs_canvas && !out->canvas) return fail(JR_E_INVALID, "shader writes a canvas but canvas is NULL");
+  if (out->display && !needs_canvas)
+    return fail(JR_E_INVALID, "display output needs a shader that writes a canvas");
   if (sh == JR_SHADER_GOURAUD && (!in->normals || !in->colours))
     return fail(JR_E_INVALID, "gouraud needs normals and colours");
   if ((sh == JR_SHADER_GOURAUD_TEXTURE || sh == JR_SHADER_PHONG || sh == JR_SHADER_PHONG_DARBOUX ||

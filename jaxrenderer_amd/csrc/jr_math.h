@@ -162,6 +162,15 @@ JR_DEV int32_t f2i_sat(float f) {
   return (int32_t)f;
 }
 
+// (clamp(0, v, 1) * 255).astype(uint8): lax.clamp keeps NaN, the product
+// stays NaN and XLA's float -> integer conversion maps NaN to 0; in range the
+// conversion truncates toward zero.
+JR_DEV uint8_t display_u8(float v) {
+  if (!(v == v)) return 0;
+  const float c = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  return (uint8_t)(int32_t)(c * 255.0f);
+}
+
 // NumPy/JAX integer index: negative values count from the end, then the
 // gather clamps to [0, n-1] (JAX's default out-of-bounds mode for reads).
 JR_DEV int32_t index_clamp(int32_t i, int32_t n) {

@@ -497,6 +497,20 @@ int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *
         } else if (fr->apply_offset) {
           out->zbuffer[pix] = out->zbuffer[pix] + fr->depth_offset;
         }
+        if (out->display && out->canvas) {
+          /* transpose_for_display((clamp(0, canvas, 1) * 255).astype(uint8)),
+           * renderer/utils.py:72-96 and notebooks/84x84/A100.ipynb cell 7 */
+          uint8_t *d = out->display + 3 * (((int64_t)s * H + (H - 1 - j)) * W + i);
+          for (int q = 0; q < 3; ++q) {
+            float v = out->canvas[3 * pix + q];
+            if (isnan(v)) {
+              d[q] = 0;
+            } else {
+              v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+              d[q] = (uint8_t)(int32_t)(v * 255.0f);
+            }
+          }
+        }
         if (out->prim_id) out->prim_id[pix] = keep ? sc->tri_begin + idx : -1;
         if (out->bary)
           for (int q = 0; q < 3; ++q) out->bary[3 * pix + q] = keep ? b[q] : 0.0f;

@@ -64,7 +64,8 @@ def _p(a: Optional[np.ndarray]) -> int:
 
 
 def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.ndarray] = None,
-               prim_id: bool = True, bary: bool = True) -> Dict[str, np.ndarray]:
+               prim_id: bool = True, bary: bool = True,
+               display: bool = False) -> Dict[str, np.ndarray]:
     L = lib()
     B, W, H = call.n_scenes, call.width, call.height
     if zbuffer is None:
@@ -81,6 +82,7 @@ def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.n
         canvas = None
     pid = np.full((B, W, H), -1, dtype=np.int32) if prim_id else None
     bar = np.zeros((B, W, H, 3), dtype=np.float32) if bary else None
+    disp = np.zeros((B, H, W, 3), dtype=np.uint8) if display else None
     keep_alive: List[np.ndarray] = []
     jin = _hip.JrInputs()
     for name, _ in _hip.JrInputs._fields_:
@@ -102,6 +104,7 @@ def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.n
     fr.tile_w, fr.tile_h = 8, 8
     out = _hip.JrOutputs()
     out.zbuffer, out.canvas, out.prim_id, out.bary = _p(zbuffer), _p(canvas), _p(pid), _p(bar)
+    out.display = _p(disp)
     scenes = (_hip.JrScene * B)(*call.scenes)
     rc = L.jr_forward_cpu(ctypes.byref(fr), ctypes.byref(jin), ctypes.cast(scenes, ctypes.c_void_p),
                           ctypes.byref(out))
@@ -114,6 +117,8 @@ def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.n
         res["prim_id"] = pid
     if bar is not None:
         res["bary"] = bar
+    if disp is not None:
+        res["display"] = disp
     return res
 
 

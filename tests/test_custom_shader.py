@@ -184,7 +184,10 @@ def test_perspective_interpolation_gpu():
     ref = render_custom(camera, CheckerShader, _buffers(W, H, "cpu"), faces, extra,
                         visibility=oracle_visibility)
     assert torch.equal(out.zbuffer.cpu(), ref.zbuffer)
-    assert float((out.targets[0].cpu() - ref.targets[0]).abs().max()) <= 1e-5
+    # the checkerboard is discontinuous in uv: a uv that torch's GPU and CPU
+    # kernels round to either side of a square's edge may flip a pixel there
+    diff = (out.targets[0].cpu() - ref.targets[0]).abs().amax(-1)
+    assert float((diff > 1e-5).float().mean()) < 1e-3
 
 
 @pytest.mark.gpu

@@ -18,8 +18,8 @@ TILES = [(8, 8), (16, 16), (8, 32), (4, 16)]
 
 
 def _check(make_calls, label, **kw):
-    ref = run_oracle_plan(make_calls())
-    got = _plan.run_plan(make_calls(), prim_id=True, bary=True, **kw)
+    ref = run_oracle_plan(make_calls(), display=True)
+    got = _plan.run_plan(make_calls(), prim_id=True, bary=True, display=True, **kw)
     torch.cuda.synchronize()
     for tag in ref:
         assert_parity(got[tag], ref[tag], label=f"{label}/{tag}")

@@ -269,3 +269,26 @@ def test_shadow_footprint_matches_ray_test():
             agree += int(shadowed == hit)
     assert total > 300
     assert agree / total > 0.97
+
+
+def test_display_output_matches_numpy_restatement():
+    """The oracle's fused display store equals numpy's clip / cast /
+    swapaxes / flip of its own canvas, including values outside [0, 1]."""
+    from oracle.oracle import run_oracle
+    from tests import scenes
+    from tests.parity import display_of
+
+    call = scenes.soup_call(GouraudShader, 4, 40, 30, 40)[0]
+    call.colour_default = (1.5, -0.25, 0.5)
+    out = run_oracle(call, display=True, prim_id=False, bary=False)
+    assert out["display"].shape == (1, 30, 40, 3)
+    np.testing.assert_array_equal(out["display"], display_of(out["canvas"]))
+    flat = out["canvas"].reshape(-1, 3)
+    assert (flat > 1).any() and (flat < 0).any()
+    # pixel (i, j) lands at row H-1-j, column i, value truncated toward zero
+    for i, j in ((5, 3), (0, 0), (39, 29), (17, 11)):
+        for q in range(3):
+            v = min(max(float(out["canvas"][0, i, j, q]), 0.0), 1.0)
+            assert out["display"][0, 30 - 1 - j, i, q] == int(np.float32(v) * np.float32(255))
+    u8 = (np.clip(out["canvas"][0], 0, 1) * np.float32(255)).astype(np.uint8)
+    np.testing.assert_array_equal(out["display"][0], transpose_for_display(u8))
