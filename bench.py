@@ -35,9 +35,10 @@ def stage_times(n_calls: int):
     from jaxrenderer_amd import _hip
 
     L = _hip.lib()
-    buf = (ctypes.c_float * (4 * n_calls))()
+    k = _hip.STAGES
+    buf = (ctypes.c_float * (k * n_calls))()
     n = L.jr_stage_times(buf, n_calls)
-    return np.array(buf[: 4 * n], dtype=np.float64).reshape(n, 4)
+    return np.array(buf[: k * n], dtype=np.float64).reshape(n, k)
 
 
 def main() -> None:
@@ -55,6 +56,10 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-scene", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
+    ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
+                    help="c3 (default): 1024 Brax-like scenes at 84x84; c4: 64-view "
+                         "differentiation of a 5000-triangle sphere at 256x256; c5: 512 "
+                         "scenes of 1k..20k triangles at 480x270, forward + backward")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy, the reference "
                          "kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE")
@@ -68,6 +73,11 @@ def main() -> None:
     import torch
 
     from jaxrenderer_amd import _hip, batch, workloads
+
+    if args.workload == "c4":
+        return run_views(args)
+    if args.workload == "c5":
+        return run_large(args)
 
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -131,14 +141,12 @@ def main() -> None:
     st = stage_times(2 * n_prof)
     shadow_ms = st[0::2].mean(axis=0)
     main_ms = st[1::2].mean(axis=0)
-    names = ["setup", "scan", "bin", "raster"]
+    names = list(_hip.STAGE_NAMES)
     stages = {f"shadow.{n}": float(v) for n, v in zip(names, shadow_ms)}
     stages.update({f"main.{n}": float(v) for n, v in zip(names, main_ms)})
     dom = max(stages, key=stages.get)
     log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
 
-    # kept pixels of the main pass (for the algorithmic byte count)
-    kept = int((zbuf_b < 1.0).sum())
 
     # forward + backward: L2 image loss against a fixed synthetic target,
     # gradients to vertices, normals, diffuse/specular maps and scene params
@@ -212,28 +220,60 @@ def main() -> None:
                          f"OpenMP over rows, {t_cpu:.1f}s"}
         log(f"cpu oracle: {n_done} scenes in {t_cpu:.1f}s")
 
-    # roofline of the dominant stage: algorithmic bytes of its launch
-    # (definitions in DESIGN.md) over its event-timed duration
+    # triangles that won at least one pixel of the main pass (their shading
+    # inputs are read by the raster stage)
+    main_pass = br.passes["main"]
+    had_ids = main_pass.prim_id is not None
+    if not had_ids:
+        main_pass.prim_id = torch.empty(zbuf_b.shape, dtype=torch.int32, device=zbuf_b.device)
+    br.render()
+    pid = main_pass.prim_id
+    n_visible = int(torch.unique(pid[pid >= 0]).numel())
+    if not had_ids:
+        main_pass.prim_id = None
+
+    # roofline of the dominant kernel: algorithmic bytes of one launch (the
+    # per-unit figures below, also listed in DESIGN.md) over its event-timed
+    # duration on the rendering stream
     n_tri = int(tris.sum())
-    n_pix_main = B * W * H
-    n_pix_shadow = B * smspec.width * smspec.height
-    alg = {
-        # read 3 slot ids + 3 position gathers (+3 attribute indices for the
-        # main pass), write one 64-byte record and a 16-byte tile range
-        "setup": lambda pass_: n_tri * ((12 + 36 + (12 if pass_ == "main" else 0)) + 80),
-        "raster": lambda pass_: (
-            (pairs[pass_] * 52 + n_pix_shadow * 4) if pass_ == "shadow" else
-            (pairs[pass_] * 52 + n_pix_main * 16
-             + kept * (64 + 3 * (4 + 12 + 12 + 8 + 12) + 4 + 8 + 12 + 4 + 4))),
-        "bin": lambda pass_: n_tri * 16 + pairs[pass_] * 4,
-        "scan": lambda pass_: (n_pix_main if pass_ == "main" else n_pix_shadow) // 64 * 12,
-    }
+    px = {"main": B * W * H, "shadow": B * smspec.width * smspec.height}
+    tiles = {tag: p.n_tiles for tag, p in br.passes.items()}
+
+    def alg_bytes(pass_: str, stage: str) -> int:
+        if stage == "clear":  # keys (8 B/pixel), tile counters, sample tables
+            return px[pass_] * 8 + tiles[pass_] * 8
+        if stage == "setup":  # 3 slot ids (+3 position indices in the main
+            # pass), 3 positions read; 64 B record + 16 B tile range written
+            return n_tri * (12 + (12 if pass_ == "main" else 0) + 36 + 80)
+        if stage == "wide":  # the wide list and ranges of wide triangles
+            return 0
+        if stage == "scan":  # count read, offset written, per tile
+            return tiles[pass_] * 8
+        if stage == "bin":  # tile range read per triangle, 4 B per binned pair
+            return n_tri * 16 + pairs[pass_] * 4
+        # raster: keys read (8 B/px), depth (+ colour) written, 52 B per
+        # binned pair (id + 48 B record), and per visible triangle of the main
+        # pass its record (64 B), face row (12 B) and three corners' indices
+        # and attributes (4 x 4 B indices + position, normal, uv: 48 B each)
+        out = px[pass_] * (4 if pass_ == "shadow" else 16)
+        shade = n_visible * (64 + 12 + 3 * 48) if pass_ == "main" else 0
+        return px[pass_] * 8 + out + pairs[pass_] * 52 + shade
+
     dpass, dstage = dom.split(".")
-    dom_bytes = alg[dstage](dpass)
+    dom_bytes = alg_bytes(dpass, dstage)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "config3_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            measured = json.load(f).get("kernels", {})
+        if dom in measured:
+            traffic = measured[dom]["hbm_bytes"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": dom, "kernel_ms": round(stages[dom], 4), "alg_bytes": int(dom_bytes)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": dom, "kernel_ms": round(stages[dom], 4), "alg_bytes": int(dom_bytes),
+                "alg_bytes_by_kernel": {k: alg_bytes(*k.split(".")) for k in stages},
+                "visible_triangles": n_visible}
 
     res = {
         "metric": "frames/s at 84x84 batch 1024 (fwd, fwd+bwd) vs per-scene launch, 1 GPU",
@@ -274,6 +314,150 @@ def main() -> None:
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def _timed(fn, steps: int, warmup: int) -> float:
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_views(args) -> None:
+    """64 ring views of a deformed 5000-triangle sphere at 256x256 (512x512
+    texture), gouraud_texture and phong: batched forward, and forward + L2
+    loss gradient + backward to vertices, normals, texture and the per-view
+    parameters, each in one HIP graph."""
+    import numpy as np
+    import torch
+
+    from jaxrenderer_amd import batch, workloads
+    from jaxrenderer_amd.autograd import build_views_call
+    from jaxrenderer_amd.shaders import (GouraudTextureExtraInput, GouraudTextureShader,
+                                         PhongTextureExtraInput, PhongTextureShader)
+    from jaxrenderer_amd.types import LightSource
+
+    f32 = np.float32
+    V, S = 64, 256
+    cams, faces, v, n, uv, col, tex = workloads.config4(seed=args.seed, n_views=V, size=S)
+    _, _, vt, _, _, _, text = workloads.config4(seed=args.seed, n_views=V, size=S,
+                                                perturbed=True)
+    light = LightSource(direction=np.array((0.3, -0.5, 0.8), f32), colour=np.ones(3, f32))
+    lines = {}
+    for shader, X in ((GouraudTextureShader, GouraudTextureExtraInput),
+                      (PhongTextureShader, PhongTextureExtraInput)):
+        extra = X(position=v, normal=n, uv=uv, light=light, texture=tex)
+        target_extra = X(position=vt, normal=n, uv=uv, light=light, texture=text)
+        call = build_views_call(cams, shader, faces, extra, S, S, init_buffers=True)
+        tcall = build_views_call(cams, shader, faces, target_extra, S, S, init_buffers=True)
+        br = batch.BatchRenderer([call], tile=(8, 8), persistent=False)
+        if args.tile == "auto":
+            br.autotune(use_cache=not args.no_autotune_cache)
+        br.calibrate()
+        target = batch.BatchRenderer([tcall], tile=(8, 8), persistent=False).render()["canvas"]
+        target = target.clone()
+        br.capture()
+        t_f = _timed(br.graph.replay, args.steps, args.warmup)
+        br.enable_backward()
+        br.capture_train(target)
+        t_fb = _timed(br.train_graph.replay, args.steps, args.warmup)
+        finite = bool(all(torch.isfinite(g).all() for g in br.grads.values()))
+        name = shader.__name__
+        lines[name] = {"fwd_views_per_s": round(V / t_f, 1), "fwd_ms": round(t_f * 1e3, 4),
+                       "fwd_bwd_views_per_s": round(V / t_fb, 1),
+                       "fwd_bwd_ms": round(t_fb * 1e3, 4),
+                       "fwd_bwd_over_fwd": round(t_f / t_fb, 3), "grads_finite": finite,
+                       "tile": list(br.passes["main"].tile)}
+        log(f"{name}: forward {t_f * 1e3:.3f} ms, forward+backward {t_fb * 1e3:.3f} ms "
+            f"for {V} views")
+    first = lines["GouraudTextureShader"]
+    print(json.dumps({
+        "metric": "views/s, 64 views 256x256 forward+backward (differentiation workload), 1 GPU",
+        "value": first["fwd_bwd_views_per_s"], "unit": "views/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": first["fwd_bwd_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "config 4: deformed UV sphere (5000 triangles), 64 ring views, "
+                               "256x256, 512x512x3 texture, L2 loss against a perturbed "
+                               "target (vertex sigma 0.02, texture sigma 0.1)",
+                   "views": V, "width": S, "height": S, "triangles": int(len(faces))},
+        "shaders": lines}), flush=True)
+
+
+def run_large(args) -> None:
+    """512 scenes of 1k..20k triangles at 480x270, phong_reflection_shadow with
+    512x512 shadow maps: batched forward, batched forward + backward (only the
+    primitive ids are kept; fragments are recomputed), peak HBM, and the plain
+    per-scene forward + backward."""
+    import torch
+
+    from jaxrenderer_amd import batch, workloads
+
+    B = min(args.scenes, 512)
+    W, H = 480, 270
+    t0 = time.time()
+    scenes, smspec = workloads.config5(n_scenes=B, seed=args.seed, width=W, height=H)
+    tris = workloads.triangle_counts(scenes)
+    log(f"generated {B} scenes ({tris.sum()} triangles, {tris.min()}..{tris.max()}) in "
+        f"{time.time() - t0:.1f}s")
+    calls = batch.plan_batch(scenes, W, H, smspec)
+    torch.cuda.reset_peak_memory_stats()
+    br = batch.BatchRenderer(calls, persistent=False)
+    if args.tile == "auto":
+        for tag, c in br.autotune(use_cache=not args.no_autotune_cache).items():
+            log(f"autotune {tag}: tile {c['tile']} persistent={c['persistent']}")
+    pairs = br.calibrate()
+    br.capture()
+    t_f = _timed(br.graph.replay, args.steps, args.warmup)
+    out = br.render()
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    target = torch.rand(out["canvas"].shape, generator=gen, device="cuda")
+    br.enable_backward()
+    br.capture_train(target)
+    t_fb = _timed(br.train_graph.replay, args.steps, args.warmup)
+    peak = torch.cuda.max_memory_allocated()
+    finite = bool(all(torch.isfinite(g).all() for g in br.grads.values()))
+    log(f"forward {t_f * 1e3:.2f} ms, forward+backward {t_fb * 1e3:.2f} ms, peak HBM "
+        f"{peak / 2**30:.2f} GiB")
+    per_scene = None
+    if not args.no_per_scene:
+        n_single = min(B, 64)
+        singles = []
+        for i in range(n_single):
+            r = batch.BatchRenderer(batch.plan_batch([scenes[i]], W, H, smspec),
+                                    persistent=False)
+            r.calibrate()
+            r.enable_backward()
+            singles.append(r)
+        tg = [target[i:i + 1] for i in range(n_single)]
+
+        def step():
+            for r, t in zip(singles, tg):
+                r.train_step(t)
+
+        t_s = _timed(step, 1, 1)
+        per_scene = n_single / t_s
+        log(f"per-scene forward+backward: {per_scene:.1f} frames/s over {n_single} scenes")
+    print(json.dumps({
+        "metric": "frames/s at 480x270 batch 512 (fwd+bwd) vs per-scene launch, 1 GPU",
+        "value": round(B / t_fb, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_fb * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "config 5: 512 Brax-like scenes of 1k..20k triangles, 480x270, "
+                               "phong_reflection_shadow, 512x512 shadow maps, forward + L2 "
+                               "backward in one HIP graph",
+                   "scenes": B, "triangles": int(tris.sum()), "width": W, "height": H},
+        "fwd_fps": round(B / t_f, 1), "fwd_ms": round(t_f * 1e3, 4),
+        "fwd_bwd_over_fwd": round(t_f / t_fb, 3), "peak_hbm_bytes": int(peak),
+        "per_scene_fwd_bwd_fps": None if per_scene is None else round(per_scene, 1),
+        "batched_over_per_scene": None if per_scene is None else round(B / t_fb / per_scene, 2),
+        "grads_finite": finite, "binned_pairs": pairs}), flush=True)
 
 
 if __name__ == "__main__":

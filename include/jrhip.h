@@ -214,10 +214,13 @@ int32_t jr_binned_pairs(const void* workspace, const jr_frame* frame,
 
 /* Instrumentation (no counterpart in the reference, whose notebooks time whole
  * jitted calls): when enabled, each jr_forward records HIP events on its own
- * stream around its four stages (setup, scan, bin, raster).
- * jr_stage_times waits for them and writes 4 floats (ms) per recorded call,
+ * stream around its JR_STAGES stages: 0 clear (k_clear), 1 setup (k_setup),
+ * 2 wide (k_wide counting), 3 scan (k_scan_*), 4 bin (k_bin and k_wide
+ * binning), 5 raster (k_raster).
+ * jr_stage_times waits for them and writes JR_STAGES floats (ms) per call,
  * oldest first, for up to max_calls calls; returns the number written and
  * forgets all recorded calls. */
+#define JR_STAGES 6
 int32_t jr_stage_timing(int32_t enable);
 int32_t jr_stage_times(float* ms, int32_t max_calls);
 

@@ -27,6 +27,10 @@ SHADER_PHONG_REFLECTION = 5
 SHADER_PHONG_REFLECTION_SHADOW = 6
 SHADER_VISIBILITY = 7
 
+# jr_stage_times: JR_STAGES timings per jr_forward call, in this order
+STAGE_NAMES = ("clear", "setup", "wide", "scan", "bin", "raster")
+STAGES = len(STAGE_NAMES)
+
 c_f = ctypes.c_float
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64

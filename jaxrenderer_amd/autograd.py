@@ -165,6 +165,75 @@ def _row_tensor(dev, camera, extra, call: RasterCall) -> torch.Tensor:
     return torch.cat(parts)[None, :]
 
 
+def build_views_call(cameras: Sequence, shader, face_indices, extra, width: int, height: int,
+                     **kw) -> RasterCall:
+    """One jr_forward call rendering the same scene through several cameras:
+    scene b is view b, every view shares the vertex, attribute and texture
+    arrays (its face rows are the same slots), so gradients of shared inputs
+    sum over the views in the backward pass.  A shadow map is repeated per
+    view (each scene reads its own)."""
+    cameras = list(cameras)
+    if not cameras:
+        raise ValueError("at least one camera is needed")
+    base = build_call(cameras[0], shader, face_indices, extra, width, height, **kw)
+    B, F = len(cameras), base.n_tris
+    scenes = []
+    for b, cam in enumerate(cameras):
+        c = _hip.JrScene.from_buffer_copy(bytes(base.scenes[0]))
+        for name in ("world_to_clip", "viewport", "world_to_eye_norm"):
+            m = np.asarray(getattr(cam, name) if not isinstance(getattr(cam, name), torch.Tensor)
+                           else getattr(cam, name).detach().cpu().numpy(), np.float32)
+            setattr(c, name, (ctypes.c_float * 16)(*m.reshape(-1)))
+        c.tri_begin, c.tri_end = b * F, (b + 1) * F
+        scenes.append(c)
+    inputs = dict(base.inputs)
+    faces = inputs["faces"]
+    inputs["faces"] = (faces.repeat(B, 1) if isinstance(faces, torch.Tensor)
+                       else np.tile(np.asarray(faces), (B, 1)))
+    if "shadow_maps" in inputs:
+        sm = inputs["shadow_maps"]
+        inputs["shadow_maps"] = (sm[None].repeat(B, 1, 1) if isinstance(sm, torch.Tensor)
+                                 else np.tile(np.asarray(sm)[None], (B, 1, 1)))
+    call = RasterCall(**{**base.__dict__, "scenes": scenes, "inputs": inputs, "n_tris": B * F})
+    return call
+
+
+def render_views_differentiable(cameras: Sequence, shader, face_indices, extra, width: int,
+                                height: int, *, zbuffer_default: float = 1.0,
+                                colour_default=(1.0, 1.0, 1.0)
+                                ) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Like render_differentiable for a list of cameras, in one batched
+    forward and one backward launch.  Returns (zbuffer [B, W, H], canvas
+    [B, W, H, 3]); gradients reach every camera's tensors and the shared
+    tensor leaves of `extra`."""
+    if not is_builtin(shader):
+        raise NotImplementedError("render_views_differentiable supports the built-in shaders")
+    if shader._jr_shader_id == _hip.SHADER_PHONG_DARBOUX:
+        raise NotImplementedError("phong_darboux has no backward pass")
+    dev = _raster.require_device()
+
+    def host(x):
+        return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x
+
+    def host_tuple(nt):
+        return type(nt)(*(host(v) if not hasattr(v, "_fields") else host_tuple(v) for v in nt))
+
+    cams_h = [type(c)(*(host(v) for v in c)) for c in cameras]
+    extra_h = host_tuple(extra)
+    call = build_views_call(cams_h, shader, host(face_indices), extra_h, width, height,
+                            init_buffers=True, zbuffer_default=zbuffer_default,
+                            colour_default=colour_default)
+    one = RasterCall(**{**call.__dict__, "scenes": call.scenes[:1]})
+    params = torch.cat([_row_tensor(dev, cam, extra, one) for cam in cameras])
+
+    def leaf(name):
+        v = getattr(extra, name, None)
+        return None if v is None else _as_tensor(v, dev).float()
+
+    return RenderFunction.apply(call, params, leaf("position"), leaf("normal"), leaf("colour"),
+                                leaf("texture"), leaf("specular_map"))
+
+
 def render_differentiable(camera, shader, face_indices, extra, width: int, height: int, *,
                           zbuffer_default: float = 1.0,
                           colour_default=(1.0, 1.0, 1.0)) -> Tuple[torch.Tensor, torch.Tensor]:
@@ -199,4 +268,5 @@ def render_differentiable(camera, shader, face_indices, extra, width: int, heigh
     return zb[0], cv[0]
 
 
-__all__ = ["RenderFunction", "render_differentiable", "scene_rows", "scenes_with_rows"]
+__all__ = ["RenderFunction", "build_views_call", "render_differentiable",
+           "render_views_differentiable", "scene_rows", "scenes_with_rows"]

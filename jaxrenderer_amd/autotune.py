@@ -58,7 +58,11 @@ def _store(path: str, d: Dict[str, dict]) -> None:
 
 
 def pass_key(tag: str, call, device: torch.device) -> str:
-    name = torch.cuda.get_device_name(device)
+    # the architecture and CU count identify the chip even where the marketing
+    # name is unavailable (it reads empty under some profilers)
+    props = torch.cuda.get_device_properties(device)
+    arch = getattr(props, "gcnArchName", "") or torch.cuda.get_device_name(device)
+    name = f"{arch.split(':')[0]}/{props.multi_processor_count}cu"
     tris = 1 << max(0, math.ceil(math.log2(max(call.n_tris, 1))))
     return (f"{name}|{tag}|shader{call.shader}|{call.width}x{call.height}|"
             f"scenes{call.n_scenes}|tris<={tris}")

@@ -288,14 +288,14 @@ class BatchRenderer:
         if main.prim_id is None:
             main.prim_id = torch.empty((c.n_scenes, c.width, c.height), dtype=torch.int32,
                                        device=dev)
-        self.grads = {
-            "d_positions": torch.zeros_like(main.inputs["positions"]),
-            "d_normals": torch.zeros_like(main.inputs["normals"]),
-            "d_textures": torch.zeros_like(main.inputs["textures"]),
-            "d_specular_maps": torch.zeros_like(main.inputs["specular_maps"]),
-            "d_scene": torch.zeros((c.n_scenes, _hip.DSCENE_FLOATS), dtype=torch.float32,
-                                   device=dev),
-        }
+        self.grads = {}
+        for name, key in (("d_positions", "positions"), ("d_normals", "normals"),
+                          ("d_colours", "colours"), ("d_textures", "textures"),
+                          ("d_specular_maps", "specular_maps")):
+            if main.inputs.get(key) is not None:
+                self.grads[name] = torch.zeros_like(main.inputs[key])
+        self.grads["d_scene"] = torch.zeros((c.n_scenes, _hip.DSCENE_FLOATS),
+                                            dtype=torch.float32, device=dev)
         need = _raster.backward_workspace_size(width=c.width, height=c.height,
                                                n_scenes=c.n_scenes, shader=c.shader,
                                                n_tris=c.n_tris)

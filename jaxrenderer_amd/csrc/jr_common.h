@@ -69,6 +69,17 @@ JR_DEV int32_t scene_of_tri(const jr_scene* __restrict__ scenes, int32_t n,
   return lo;
 }
 
+// scene_of_tri for a wavefront whose triangles mostly share one scene: the
+// first active lane's scene is found with wavefront-uniform (scalar) loads,
+// and only lanes outside its triangle range search on their own.
+JR_DEV int32_t scene_of_tri_wave(const jr_scene* __restrict__ scenes, int32_t n,
+                                 int32_t t) {
+  const int32_t t0 = __builtin_amdgcn_readfirstlane(t);
+  const int32_t s0 = __builtin_amdgcn_readfirstlane(scene_of_tri(scenes, n, t0));
+  if (t >= scenes[s0].tri_begin && t < scenes[s0].tri_end) return s0;
+  return scene_of_tri(scenes, n, t);
+}
+
 // PerPrimitive.create (renderer/pipeline.py:76-113) for face row t: clip
 // coordinates of its three slots, the [x, y, w] matrix, its determinant,
 // keep / front-facing flags and the inverse.

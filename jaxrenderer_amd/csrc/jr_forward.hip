@@ -38,12 +38,61 @@
 namespace jr {
 
 // Conservative pixel range of a triangle.  The pixel test accepts (x, y) only
-// when every float edge value is >= 0; float rounding keeps each computed edge
-// within 3u(|x a| + |y b| + |c|) of its exact value (u = 2^-24), so every
-// accepted pixel lies in the region {a x + b y + c >= -delta_k} with
-// delta_k = 8u(X|a| + Y|b| + |c|).  That region, intersected with the
-// viewport's sample rectangle, is clipped in double precision and its bounding
-// box, widened by one pixel, gives the range (inclusive pixel indices).
+// when every float edge value is >= 0.  The float sample position and the
+// three rounded operations keep each computed edge within about
+// 5.1u(X|a| + Y|b| + |c|) of the exact edge at the exact sample position
+// (u = 2^-24), so every accepted pixel's exact sample lies in the region
+// R = {a_k x + b_k y + c_k >= -delta_k} with delta_k = 8u(X|a| + Y|b| + |c|).
+// The range is the bounding box of R within the viewport's sample rectangle,
+// mapped to pixel indices with a 1/64-pixel allowance for the double-precision
+// rounding of the box itself.
+//
+// Common case: the three shifted lines meet pairwise in three points that
+// each satisfy the third inequality with room to spare; R is then exactly the
+// triangle they span, whose bounding box needs three 2x2 solves.  Otherwise
+// (R unbounded, empty or ill-conditioned) the viewport rectangle is clipped
+// by the three half-planes in double precision.
+JR_DEV bool shifted_triangle_box(const double a[3], const double b[3],
+                                 const double c[3], double* xmin, double* xmax,
+                                 double* ymin, double* ymax) {
+  // g bounds the relative rounding of a few double operations (2^-53 each)
+  constexpr double g = 8.0 / 9007199254740992.0;
+  double lo_x = INFINITY, hi_x = -INFINITY, lo_y = INFINITY, hi_y = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int p = k, q = (k + 1) % 3, r = (k + 2) % 3;
+    const double det = a[p] * b[q] - a[q] * b[p];
+    const double scale = (fabs(a[p]) + fabs(b[p])) * (fabs(a[q]) + fabs(b[q]));
+    if (!(fabs(det) > 1e-6 * scale)) return false;
+    const double inv = 1.0 / det;
+    const double nx = b[p] * c[q] - b[q] * c[p];
+    const double ny = a[q] * c[p] - a[p] * c[q];
+    const double x = nx * inv, y = ny * inv;
+    // absolute error bounds of x and y from the rounding of det, the
+    // numerators and the division
+    const double ex = g * ((fabs(b[p] * c[q]) + fabs(b[q] * c[p])) * fabs(inv) +
+                           fabs(x) * (fabs(a[p] * b[q]) + fabs(a[q] * b[p])) * fabs(inv) +
+                           fabs(x));
+    const double ey = g * ((fabs(a[q] * c[p]) + fabs(a[p] * c[q])) * fabs(inv) +
+                           fabs(y) * (fabs(a[p] * b[q]) + fabs(a[q] * b[p])) * fabs(inv) +
+                           fabs(y));
+    const double v = a[r] * x + b[r] * y + c[r];
+    const double mag = fabs(a[r] * x) + fabs(b[r] * y) + fabs(c[r]);
+    // the true intersection must satisfy the third inequality
+    if (!(v > fabs(a[r]) * ex + fabs(b[r]) * ey + g * mag)) return false;
+    lo_x = fmin(lo_x, x - ex);
+    hi_x = fmax(hi_x, x + ex);
+    lo_y = fmin(lo_y, y - ey);
+    hi_y = fmax(hi_y, y + ey);
+  }
+  if (!(lo_x <= hi_x) || !(lo_y <= hi_y)) return false;
+  *xmin = lo_x;
+  *xmax = hi_x;
+  *ymin = lo_y;
+  *ymax = hi_y;
+  return true;
+}
+
 JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
                         bool sane, const float* vp, const Frame& f) {
   Rect full = {0, f.W - 1, 0, f.H - 1};
@@ -52,52 +101,68 @@ JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
   const double Y0 = pix_y(0, vp), Y1 = pix_y(f.H - 1, vp);
   if (!(X0 <= X1) || !(Y0 <= Y1)) return full;
   const double XM = fmax(fabs(X0), fabs(X1)), YM = fmax(fabs(Y0), fabs(Y1));
-  double px[8], py[8], qx[8], qy[8];
-  px[0] = X0; py[0] = Y0;
-  px[1] = X1; py[1] = Y0;
-  px[2] = X1; py[2] = Y1;
-  px[3] = X0; py[3] = Y1;
-  int n = 4;
   const float4 rs[3] = {r0, r1, r2};
-  for (int k = 0; k < 3 && n > 0; ++k) {
-    const double a = rs[k].x, b = rs[k].y, c = rs[k].z;
-    const double delta =
-        (8.0 / 16777216.0) * (XM * fabs(a) + YM * fabs(b) + fabs(c)) + 1e-300;
-    int m = 0;
-    for (int v = 0; v < n; ++v) {
-      const int w = (v + 1 == n) ? 0 : v + 1;
-      const double dv = a * px[v] + b * py[v] + c + delta;
-      const double dw = a * px[w] + b * py[w] + c + delta;
-      if (dv >= 0.0) {
-        qx[m] = px[v];
-        qy[m] = py[v];
-        ++m;
-      }
-      if ((dv >= 0.0) != (dw >= 0.0)) {
-        const double t = dv / (dv - dw);
-        qx[m] = px[v] + t * (px[w] - px[v]);
-        qy[m] = py[v] + t * (py[w] - py[v]);
-        ++m;
-      }
-    }
-    n = m;
-    for (int v = 0; v < n; ++v) {
-      px[v] = qx[v];
-      py[v] = qy[v];
-    }
+  double A[3], B[3], C[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    A[k] = rs[k].x;
+    B[k] = rs[k].y;
+    C[k] = (double)rs[k].z +
+           (8.0 / 16777216.0) * (XM * fabs(A[k]) + YM * fabs(B[k]) + fabs((double)rs[k].z)) +
+           1e-300;
   }
   Rect r = {1, 0, 1, 0};
-  if (n == 0) return r;
-  double xmin = px[0], xmax = px[0], ymin = py[0], ymax = py[0];
-  for (int v = 1; v < n; ++v) {
-    xmin = fmin(xmin, px[v]);
-    xmax = fmax(xmax, px[v]);
-    ymin = fmin(ymin, py[v]);
-    ymax = fmax(ymax, py[v]);
+  double xmin, xmax, ymin, ymax;
+  if (shifted_triangle_box(A, B, C, &xmin, &xmax, &ymin, &ymax)) {
+    xmin = fmax(xmin, X0);
+    xmax = fmin(xmax, X1);
+    ymin = fmax(ymin, Y0);
+    ymax = fmin(ymax, Y1);
+    if (!(xmin <= xmax) || !(ymin <= ymax)) return r;
+  } else {
+    double px[8], py[8], qx[8], qy[8];
+    px[0] = X0; py[0] = Y0;
+    px[1] = X1; py[1] = Y0;
+    px[2] = X1; py[2] = Y1;
+    px[3] = X0; py[3] = Y1;
+    int n = 4;
+    for (int k = 0; k < 3 && n > 0; ++k) {
+      int m = 0;
+      for (int v = 0; v < n; ++v) {
+        const int w = (v + 1 == n) ? 0 : v + 1;
+        const double dv = A[k] * px[v] + B[k] * py[v] + C[k];
+        const double dw = A[k] * px[w] + B[k] * py[w] + C[k];
+        if (dv >= 0.0) {
+          qx[m] = px[v];
+          qy[m] = py[v];
+          ++m;
+        }
+        if ((dv >= 0.0) != (dw >= 0.0)) {
+          const double t = dv / (dv - dw);
+          qx[m] = px[v] + t * (px[w] - px[v]);
+          qy[m] = py[v] + t * (py[w] - py[v]);
+          ++m;
+        }
+      }
+      n = m;
+      for (int v = 0; v < n; ++v) {
+        px[v] = qx[v];
+        py[v] = qy[v];
+      }
+    }
+    if (n == 0) return r;
+    xmin = px[0], xmax = px[0], ymin = py[0], ymax = py[0];
+    for (int v = 1; v < n; ++v) {
+      xmin = fmin(xmin, px[v]);
+      xmax = fmax(xmax, px[v]);
+      ymin = fmin(ymin, py[v]);
+      ymax = fmax(ymax, py[v]);
+    }
   }
   const double vx = vp[0], vx3 = vp[3], vy = vp[5], vy3 = vp[7];
-  double i0 = floor(xmin * vx + vx3) - 1.0, i1 = ceil(xmax * vx + vx3) + 1.0;
-  double j0 = floor(ymin * vy + vy3) - 1.0, j1 = ceil(ymax * vy + vy3) + 1.0;
+  constexpr double kSlack = 1.0 / 64.0;
+  double i0 = ceil(xmin * vx + vx3 - kSlack), i1 = floor(xmax * vx + vx3 + kSlack);
+  double j0 = ceil(ymin * vy + vy3 - kSlack), j1 = floor(ymax * vy + vy3 + kSlack);
   i0 = fmax(i0, 0.0);
   j0 = fmax(j0, 0.0);
   i1 = fmin(i1, (double)(f.W - 1));
@@ -176,78 +241,138 @@ JR_DEV bool has_nan_column(const float4& r) {
   return (r.x != r.x) || (r.y != r.y) || (r.z != r.z);
 }
 
-// Clears the tile counters (n32 int32) and sets the per-pixel keys (n64
-// uint64) to "no candidate".
+// Clears the tile counters (n32 int32), sets the per-pixel keys (n64
+// uint64) to "no candidate" and fills every scene's sample table: the NDC
+// coordinates pix_x(i) for i < W followed by pix_y(j) for j < H.
 __global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __restrict__ p64,
-                        int64_t n64) {
+                        int64_t n64, const jr_scene* __restrict__ scenes, Frame f,
+                        float* __restrict__ samples) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n32; i += stride)
-    p32[i] = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n64; i += stride)
-    p64[i] = ~0ull;
+  const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t i = first; i < n32; i += stride) p32[i] = 0;
+  for (int64_t i = first; i < n64; i += stride) p64[i] = ~0ull;
+  const int32_t per = f.W + f.H;
+  for (int64_t i = first; i < (int64_t)f.n_scenes * per; i += stride) {
+    const int32_t s = (int32_t)(i / per), q = (int32_t)(i - (int64_t)s * per);
+    const float* vp = scenes[s].viewport;
+    samples[i] = q < f.W ? pix_x(q, vp) : pix_y(q - f.W, vp);
+  }
 }
 
 // Stage 1: vertex positions -> clip space, PerPrimitive.create; small
 // triangles are resolved into the key buffer here, large ones counted into
 // the tiles they may cover.
-__global__ __launch_bounds__(256) void k_setup(
+//
+// The pixels of a wavefront's small triangles are spread over its 64 lanes:
+// a wavefront prefix sum of the triangles' pixel-box areas numbers all of
+// their (triangle, pixel) pairs, and each pass of the loop gives one pair to
+// every lane (the owning triangle found by a binary search over the prefix
+// sums in LDS).  A wavefront therefore takes sum(areas) / 64 steps instead of
+// max(area) steps of one lane per triangle.
+constexpr int kSetupThreads = 256;
+
+struct SmallStage {
+  float4 r[3][kSetupThreads];
+  int4 box[kSetupThreads];  // tx0, ty0, box height, scene
+  int32_t tri[kSetupThreads];
+  int32_t off[kSetupThreads];  // exclusive prefix sum of the areas per wavefront
+};
+
+__global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     float4* __restrict__ rec, Rect* __restrict__ rects,
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
-    int32_t* __restrict__ wide, int32_t* __restrict__ wide_count) {
+    int32_t* __restrict__ wide, int32_t* __restrict__ wide_count,
+    const float* __restrict__ samples) {
+  __shared__ SmallStage sm;
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= in.n_tris) return;
-  const int32_t s = scene_of_tri(scenes, f.n_scenes, t);
-  const jr_scene& sc = scenes[s];
-  TriSetup ts;
-  tri_setup(in, sc, t, ts);
-  const bool keep = ts.keep, front = ts.front, sane = ts.sane;
-  const float det = ts.det;
-  uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
-  const float4 r0 = rec_col(ts, 0);
-  const float4 r1 = rec_col(ts, 1);
-  const float4 r2 = rec_col(ts, 2);
-  rec[4 * (int64_t)t + 0] = r0;
-  rec[4 * (int64_t)t + 1] = r1;
-  rec[4 * (int64_t)t + 2] = r2;
-  rec[4 * (int64_t)t + 3] = make_float4(det, __uint_as_float(flags), 0.0f, 0.0f);
-  Rect r = {1, 0, 1, 0};
-  Rect px = r;
-  // a NaN coefficient makes that edge value NaN at every pixel: never inside
-  const bool nan_col = has_nan_column(r0) || has_nan_column(r1) || has_nan_column(r2);
-  if (keep && front && !nan_col) px = pixel_range(r0, r1, r2, sane, sc.viewport, f);
-  const int64_t area = (px.tx0 > px.tx1)
-      ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
-  if (area > kSmallPixels) r = tiles_of(px, f);
-  rects[t] = r;
-  if (area > 0 && area <= kSmallPixels) {
-    // the tile raster's exact per-pixel arithmetic, pixel by pixel
-    const float* vp = sc.viewport;
-    const float vp22 = vp[10], vp23 = vp[11];
-    uint64_t* kb = keys + (int64_t)s * f.W * f.H;
-    for (int32_t i = px.tx0; i <= px.tx1; ++i) {
-      const float x = pix_x(i, vp);
-      for (int32_t j = px.ty0; j <= px.ty1; ++j) {
-        const float y = pix_y(j, vp);
-        const float e0 = edge(r0, x, y), e1 = edge(r1, x, y), e2 = edge(r2, x, y);
-        if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
-          const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
-          atomicMin((unsigned long long*)(kb + (int64_t)i * f.H + j),
-                    (unsigned long long)pack_key(key, t));
-        }
-      }
+  const bool valid = t < in.n_tris;
+  const int lane = threadIdx.x & 63;
+  const int wbase = threadIdx.x & ~63;
+  int32_t n_small = 0;
+  if (valid) {
+    const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
+    const jr_scene& sc = scenes[s];
+    TriSetup ts;
+    tri_setup(in, sc, t, ts);
+    const bool keep = ts.keep, front = ts.front, sane = ts.sane;
+    const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
+    const float4 r0 = rec_col(ts, 0);
+    const float4 r1 = rec_col(ts, 1);
+    const float4 r2 = rec_col(ts, 2);
+    Rect r = {1, 0, 1, 0};
+    rec[4 * (int64_t)t + 0] = r0;
+    rec[4 * (int64_t)t + 1] = r1;
+    rec[4 * (int64_t)t + 2] = r2;
+    rec[4 * (int64_t)t + 3] = make_float4(ts.det, __uint_as_float(flags), 0.0f, 0.0f);
+    Rect px = r;
+    // a NaN coefficient makes that edge value NaN at every pixel: never inside
+    const bool nan_col = has_nan_column(r0) || has_nan_column(r1) || has_nan_column(r2);
+    if (keep && front && !nan_col) px = pixel_range(r0, r1, r2, sane, sc.viewport, f);
+    const int64_t area = (px.tx0 > px.tx1)
+        ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
+    if (area > kSmallPixels) r = tiles_of(px, f);
+    rects[t] = r;
+    if (area > 0 && area <= kSmallPixels) {
+      n_small = (int32_t)area;
+      sm.r[0][threadIdx.x] = r0;
+      sm.r[1][threadIdx.x] = r1;
+      sm.r[2][threadIdx.x] = r2;
+      sm.box[threadIdx.x] = make_int4(px.tx0, px.ty0, px.ty1 - px.ty0 + 1, s);
+      sm.tri[threadIdx.x] = t;
+    }
+    const int64_t n_t =
+        (r.tx0 > r.tx1) ? 0 : (int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
+    if (n_t > kWideTiles) {
+      wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
+    } else {
+      int32_t* cnt = counts + (int64_t)s * f.T;
+      for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
+        for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
+          if (tile_may_cover(r0, r1, r2, sane, sc.viewport, f, tx, ty))
+            atomicAdd(cnt + tx * f.tiles_y + ty, 1);
     }
   }
-  const int64_t n_t = (r.tx0 > r.tx1) ? 0 : (int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
-  if (n_t > kWideTiles) {
-    wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
-    return;
+  int32_t incl = n_small;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
   }
-  int32_t* cnt = counts + (int64_t)s * f.T;
-  for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
-    for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
-      if (tile_may_cover(r0, r1, r2, sane, sc.viewport, f, tx, ty))
-        atomicAdd(cnt + tx * f.tiles_y + ty, 1);
+  const int32_t total = __shfl(incl, 63, 64);
+  sm.off[threadIdx.x] = incl - n_small;
+  __syncthreads();
+  // the tile raster's exact per-pixel arithmetic for every (triangle, pixel)
+  for (int32_t base = 0; base < total; base += 64) {
+    const int32_t item = base + lane;
+    if (item >= total) break;
+    int32_t lo = 0, hi = 63;  // last lane whose first item is <= item
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (sm.off[wbase + mid] <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int o = wbase + lo;
+    const int4 box = sm.box[o];
+    const int32_t q = item - sm.off[o];
+    // q / h for q < 64, h <= 64: a float estimate, corrected to the exact quotient
+    int32_t di = (int32_t)((float)q / (float)box.z);
+    if (di * box.z > q) --di;
+    if ((di + 1) * box.z <= q) ++di;
+    const int32_t i = box.x + di, j = box.y + (q - di * box.z);
+    const float4 q0 = sm.r[0][o], q1 = sm.r[1][o], q2 = sm.r[2][o];
+    const float* sx = samples + (int64_t)box.w * (f.W + f.H);
+    const float x = sx[i], y = sx[f.W + j];
+    const float e0 = edge(q0, x, y), e1 = edge(q1, x, y), e2 = edge(q2, x, y);
+    if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
+      const float* vp = scenes[box.w].viewport;
+      const uint32_t key = depth_key(win_depth(q0, q1, q2, e0, e1, e2, vp[10], vp[11]));
+      atomicMin((unsigned long long*)(keys + ((int64_t)box.w * f.W + i) * f.H + j),
+                (unsigned long long)pack_key(key, sm.tri[o]));
+    }
+  }
 }
 
 // Counts (BIN = false) or bins (BIN = true) the wide triangles: one wavefront
@@ -266,7 +391,7 @@ __global__ __launch_bounds__(256) void k_wide(
   for (int64_t w = wave; w < n; w += n_waves) {
     const int32_t t = wide[w];
     const Rect r = rects[t];
-    const int32_t s = scene_of_tri(scenes, f.n_scenes, t);
+    const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
     const float* vp = scenes[s].viewport;
     const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
                  r2 = rec[4 * (int64_t)t + 2];
@@ -368,7 +493,7 @@ __global__ __launch_bounds__(256) void k_bin(
   const Rect r = rects[t];
   if (r.tx0 > r.tx1) return;
   if ((int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1) > kWideTiles) return;  // k_wide
-  const int32_t s = scene_of_tri(scenes, f.n_scenes, t);
+  const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
   const float* vp = scenes[s].viewport;
   const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
                r2 = rec[4 * (int64_t)t + 2];
@@ -396,6 +521,7 @@ struct RasterArgs {
   const uint64_t* keys;
   int32_t* work_counter;
   int32_t chunk;  // tiles claimed per work-queue step (persistent form)
+  int32_t xcd_scenes;  // one-group-per-tile form: scenes kept on one XCD each
   int32_t init_buffers, apply_offset;
   float zdef, cdef0, cdef1, cdef2, depth_offset;
 };
@@ -553,11 +679,26 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by group id, and each
+// XCD has its own L2.  With many scenes, group b takes a tile of a scene
+// s == b (mod 8), so every tile of a scene runs on one XCD and the scene's
+// triangle records and attributes are fetched into one L2 only.  The grid is
+// 8 * ceil(n_scenes / 8) * T groups; the padding groups exit at once.
+constexpr int kXcds = 8;
+
 template <int SHADER, int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void k_raster(RasterArgs a) {
   __shared__ float4 lds_rec[3 * TW * TH];
   __shared__ int32_t lds_id[TW * TH];
-  raster_tile<SHADER, TW, TH>(a, blockIdx.x, lds_rec, lds_id);
+  int64_t gt = blockIdx.x;
+  if (a.xcd_scenes) {
+    const int64_t b = blockIdx.x;
+    const int64_t k = b / kXcds;
+    const int64_t s = (k / a.f.T) * kXcds + (b - k * kXcds);
+    if (s >= a.f.n_scenes) return;
+    gt = s * a.f.T + (k - (k / a.f.T) * a.f.T);
+  }
+  raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
 }
 
 // Persistent form: a fixed grid drains the (scene, tile) queue.  Every group
@@ -589,11 +730,11 @@ thread_local std::string g_err;
 // after raster); jr_stage_times reads them back.
 std::mutex g_timing_mu;
 bool g_timing = false;
-std::vector<std::array<hipEvent_t, 5>> g_timed;
+std::vector<std::array<hipEvent_t, JR_STAGES + 1>> g_timed;
 
 struct StageTimer {
   hipStream_t st;
-  std::array<hipEvent_t, 5> ev{};
+  std::array<hipEvent_t, JR_STAGES + 1> ev{};
   bool on = false;
   explicit StageTimer(hipStream_t s) : st(s) {
     std::lock_guard<std::mutex> g(g_timing_mu);
@@ -622,7 +763,8 @@ constexpr int64_t kAlign = 256;
 int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
-  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, bins, total;
+  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, samples, bins,
+      total;
   int64_t n_tiles, nblk, n_pix;
 };
 
@@ -643,6 +785,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.n_pix = (int64_t)fr->width * fr->height * fr->n_scenes;
   L.keys = o;       o += align_up(8 * L.n_pix);
   L.wide = o;       o += align_up(4 * (int64_t)std::max(n_tris, 1));
+  L.samples = o;    o += align_up(4 * (int64_t)(fr->width + fr->height) * fr->n_scenes);
   L.bins = o;       o += align_up(4 * std::max<int64_t>(fr->bin_capacity, 1));
   L.total = o;
   return L;
@@ -680,8 +823,13 @@ void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
     hipLaunchKernelGGL((k_raster_persistent<SHADER, TW, TH>), dim3((unsigned)grid),
                        dim3(TW * TH), 0, st, b, n_items);
   } else {
-    hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)n_items),
-                       dim3(TW * TH), 0, st, a);
+    RasterArgs b = a;
+    int64_t grid = n_items;
+    b.xcd_scenes = a.f.n_scenes >= 8 * kXcds;
+    if (b.xcd_scenes)
+      grid = (int64_t)kXcds * ((a.f.n_scenes + kXcds - 1) / kXcds) * a.f.T;
+    hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)grid),
+                       dim3(TW * TH), 0, st, b);
   }
 }
 
@@ -800,6 +948,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   int32_t* bins = reinterpret_cast<int32_t*>(ws + L.bins);
   uint64_t* keys = reinterpret_cast<uint64_t*>(ws + L.keys);
   int32_t* wide = reinterpret_cast<int32_t*>(ws + L.wide);
+  float* samples = reinterpret_cast<float*>(ws + L.samples);
   int32_t* wide_count = misc + 2;
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
@@ -808,31 +957,35 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   // with one fill kernel (a kernel rather than hipMemsetAsync so the sequence
   // can be graph-captured)
   const int64_t nz = (L.offsets - L.counts) / 4;
-  const int64_t nclr = std::max(nz, L.n_pix);
+  const int64_t nclr = std::max(std::max(nz, L.n_pix),
+                                (int64_t)(f.W + f.H) * f.n_scenes);
   hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<int64_t>((nclr + 255) / 256, 8192)),
-                     dim3(256), 0, st, counts, nz, keys, L.n_pix);
+                     dim3(256), 0, st, counts, nz, keys, L.n_pix, scenes, f, samples);
+  timer.mark(1);
   const int n_tris = in->n_tris;
-  if (n_tris > 0) {
-    hipLaunchKernelGGL(k_setup, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rec, rects, counts, keys, wide, wide_count);
+  if (n_tris > 0)
+    hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
+                       dim3(kSetupThreads), 0, st, *in,
+                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples);
+  timer.mark(2);
+  if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
                        rects, wide, wide_count, counts, offsets, cursor, bins);
-  }
-  timer.mark(1);
+  timer.mark(3);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
                      dim3(kScanThreads), 0, st, counts, offsets, block_sums, L.n_tiles);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, block_sums,
                      L.nblk, misc);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
                      dim3(kScanThreads), 0, st, offsets, block_sums, L.n_tiles);
-  timer.mark(2);
+  timer.mark(4);
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
                        scenes, f, rec, rects, offsets, cursor, bins);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
                        rects, wide, wide_count, counts, offsets, cursor, bins);
   }
-  timer.mark(3);
+  timer.mark(5);
   RasterArgs a;
   a.in = *in;
   a.scenes = scenes;
@@ -845,6 +998,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.keys = keys;
   a.work_counter = misc + 1;
   a.chunk = 1;
+  a.xcd_scenes = 0;
   a.init_buffers = fr->init_buffers;
   a.apply_offset = fr->apply_offset;
   a.zdef = fr->zbuffer_default;
@@ -855,12 +1009,12 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   if (n_tris == 0) {
     // no primitive: every pixel keeps its old value (plus the offset)
     if (!fr->init_buffers && !fr->apply_offset) {
-      timer.mark(4);
+      timer.mark(6);
       return JR_OK;
     }
   }
   launch_raster(fr->shader, a, L.n_tiles, fr->persistent != 0, st);
-  timer.mark(4);
+  timer.mark(6);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(JR_E_HIP, std::string("launch failed: ") + hipGetErrorString(err));
   return JR_OK;
@@ -873,7 +1027,7 @@ int32_t jr_stage_timing(int32_t enable) {
 }
 
 int32_t jr_stage_times(float* ms, int32_t max_calls) {
-  std::vector<std::array<hipEvent_t, 5>> calls;
+  std::vector<std::array<hipEvent_t, JR_STAGES + 1>> calls;
   {
     std::lock_guard<std::mutex> g(g_timing_mu);
     calls.swap(g_timed);
@@ -881,12 +1035,12 @@ int32_t jr_stage_times(float* ms, int32_t max_calls) {
   int32_t n = 0;
   for (auto& ev : calls) {
     if (n < max_calls && ms) {
-      (void)hipEventSynchronize(ev[4]);
-      for (int k = 0; k < 4; ++k) {
+      (void)hipEventSynchronize(ev[JR_STAGES]);
+      for (int k = 0; k < JR_STAGES; ++k) {
         float t = -1.0f;
         if (ev[k] && ev[k + 1] && hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess)
           t = -1.0f;
-        ms[4 * n + k] = t;
+        ms[JR_STAGES * n + k] = t;
       }
       ++n;
     }

@@ -8,6 +8,10 @@
   (8 x 16 segments) and spheres (12 x 24 segments) at poses U[-3,3]^3 with
   uniform random orientations and 8x8 textures -- rendered at 84x84 with
   phong_reflection_shadow and a 256x256 shadow map.
+* config 4: a deformed 5000-triangle sphere seen by 64 ring cameras at
+  256x256 with a 512x512 texture, for differentiation.
+* config 5: config-3-like scenes of 1k..20k triangles at 480x270 with a
+  512x512 shadow map.
 
 Everything is drawn from numpy.random.Generator(PCG64(seed)).
 """
@@ -151,8 +155,82 @@ def config2(seed: int = 0, width: int = 960, height: int = 540) -> BatchScene:
                       light=LightParameters())
 
 
+def dense_brax_like_objects(rng, target_tris: int) -> List[ModelObject]:
+    """A ground plane plus boxes, capsules and spheres of finer tessellation,
+    added until the scene holds at least `target_tris` triangles."""
+    spec8 = np.full((8, 8), 2.0, f32)
+    objs = [ModelObject(model=plane_model(10.0, rng.random((8, 8, 3)).astype(f32), spec8),
+                        transform=np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, -3.5],
+                                            [0, 0, 0, 1]], f32))]
+    total = 2
+    while total < target_tris:
+        kind = int(rng.integers(0, 3))
+        tex = rng.random((8, 8, 3)).astype(f32)
+        if kind == 0:
+            he = rng.uniform(0.2, 0.6, size=3).astype(f32)
+            model = create_cube(he, np.ones(2, f32), tex, spec8)
+        else:
+            n_lat = int(rng.choice([12, 16, 24, 32]))
+            r = float(rng.uniform(0.15, 0.6))
+            hh = float(rng.uniform(0.2, 0.5)) if kind == 1 else 0.0
+            model = mesh_model(*uv_sphere(n_lat, 2 * n_lat, r, hh), tex, spec8)
+        t = np.identity(4, dtype=f32)
+        t[:3, :3] = transform_matrix_from_rotation(random_quaternion(rng))
+        t[:3, 3] = rng.uniform(-3.0, 3.0, size=3).astype(f32)
+        objs.append(ModelObject(model=model, transform=t))
+        total += len(model.faces)
+    return objs
+
+
+def config5(n_scenes: int = 512, seed: int = 0, width: int = 480, height: int = 270,
+            shadow_size: int = 512, min_tris: int = 1000, max_tris: int = 20000
+            ) -> Tuple[List[BatchScene], ShadowMapSpec]:
+    """Batched forward + backward workload: Brax-like scenes of 1k..20k
+    triangles (target drawn log-uniformly per scene) at 480x270 with
+    phong_reflection_shadow and a 512x512 shadow map."""
+    rng = _rng(seed)
+    scenes = []
+    for i in range(n_scenes):
+        target = int(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris))))
+        model = merge_objects(dense_brax_like_objects(rng, target))
+        scenes.append(BatchScene(model=model, camera=ring_camera(i, n_scenes, width, height),
+                                 light=NOTEBOOK_LIGHT))
+    return scenes, ShadowMapSpec(width=shadow_size, height=shadow_size)
+
+
+def config4(seed: int = 0, n_views: int = 64, size: int = 256, tex_size: int = 512,
+            n_lat: int = 50, n_lon: int = 50, perturbed: bool = False):
+    """Differentiation workload: a deformed UV sphere (n_lat x n_lon quads,
+    5000 triangles by default) seen by `n_views` cameras on a ring of radius 3,
+    size x size pixels, with a tex_size^2 x 3 diffuse texture and per-vertex
+    colours.  `perturbed` gives the target scene: vertex noise sigma 0.02 and
+    texture noise sigma 0.1 drawn from the same seed stream.  Returns
+    (cameras, faces, positions, normals, uvs, colours, texture)."""
+    rng = _rng(seed)
+    noise = rng.normal(size=(n_lat + 1, n_lon + 1))
+    for _ in range(4):
+        noise = (noise + np.roll(noise, 1, 1) + np.roll(noise, -1, 1)) / 3.0
+        noise[1:-1] = (noise[1:-1] + noise[:-2] + noise[2:]) / 3.0
+    radial = 1.0 + 0.05 * noise / max(noise.std(), 1e-6)
+    v, n, uv, f = uv_sphere(n_lat, n_lon, 1.0, 0.0, radial)
+    tex = rng.random((tex_size, tex_size, 3)).astype(f32)
+    colours = rng.uniform(0.3, 1.0, size=(len(v), 3)).astype(f32)
+    if perturbed:
+        v = (v + rng.normal(0.0, 0.02, size=v.shape)).astype(f32)
+        tex = np.clip(tex + rng.normal(0.0, 0.1, size=tex.shape), 0.0, 1.0).astype(f32)
+    cams = []
+    for i in range(n_views):
+        a = 2.0 * math.pi * i / n_views
+        cp = CameraParameters(viewWidth=size, viewHeight=size, hfov=45.0, vfov=45.0,
+                              position=(3.0 * math.cos(a), 3.0 * math.sin(a), 0.8),
+                              target=(0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0))
+        cams.append(Renderer.create_camera_from_parameters(cp))
+    return cams, f, v, n, uv, colours, tex
+
+
 def triangle_counts(scenes: Sequence[BatchScene]) -> np.ndarray:
     return np.array([len(s.model.faces) for s in scenes])
 
 
-__all__ = ["config2", "config3", "brax_like_objects", "uv_sphere", "NOTEBOOK_LIGHT"]
+__all__ = ["config2", "config3", "config4", "config5", "brax_like_objects",
+           "dense_brax_like_objects", "uv_sphere", "NOTEBOOK_LIGHT"]

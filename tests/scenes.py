@@ -236,3 +236,56 @@ def soup_inputs(shader: type, seed: int, width: int = 48, height: int = 40, n_tr
     sh = Shadow(shadow_map=None, strength=np.array((0.6, 0.5, 0.4), f32), camera=scam)
     return camera, faces, PhongReflectionShadowTextureExtraInput(**common, shadow=sh,
                                                                  camera=camera), shadow_call
+
+
+def tiny_soup_call(seed: int, width: int = 96, height: int = 64, n_tris: int = 3000,
+                   shader: type = DepthShader) -> List[RasterCall]:
+    """Edge cases of the pixel-range culling: sub-pixel and few-pixel
+    triangles, slivers, and (through an orthographic camera whose clip x, y are
+    the world x, y) triangles whose vertices and edges sit exactly on pixel
+    sample positions, so that edge values of exactly 0 decide coverage."""
+    rng = np.random.default_rng(seed)
+    W, H = width, height
+    # NDC sample position of pixel i is (i - W/2) / (W/2), exactly representable
+    sx = lambda i: (np.asarray(i, f32) - f32(W / 2)) / f32(W / 2)  # noqa: E731
+    sy = lambda j: (np.asarray(j, f32) - f32(H / 2)) / f32(H / 2)  # noqa: E731
+    n_grid = n_tris // 2
+    i0 = rng.integers(-2, W + 2, size=n_grid)
+    j0 = rng.integers(-2, H + 2, size=n_grid)
+    di = rng.integers(-3, 4, size=(n_grid, 2))
+    dj = rng.integers(-3, 4, size=(n_grid, 2))
+    grid = np.zeros((n_grid, 3, 3), f32)
+    grid[:, 0, 0], grid[:, 0, 1] = sx(i0), sy(j0)
+    grid[:, 1, 0], grid[:, 1, 1] = sx(i0 + di[:, 0]), sy(j0 + dj[:, 0])
+    grid[:, 2, 0], grid[:, 2, 1] = sx(i0 + di[:, 1]), sy(j0 + dj[:, 1])
+    # half of those moved by a fraction of a pixel, some by one float ulp
+    frac = rng.random(n_grid) < 0.5
+    grid[frac, :, :2] += rng.choice([-0.5, 0.25, 1e-7, -1e-7], size=(frac.sum(), 3, 2)).astype(
+        f32) * f32(2.0 / W)
+    grid[:, :, 2] = rng.uniform(-0.9, 0.9, size=(n_grid, 3)).astype(f32)
+    n_free = n_tris - n_grid
+    c = rng.uniform(-1.05, 1.05, size=(n_free, 1, 3)).astype(f32)
+    size = np.exp(rng.uniform(np.log(0.002), np.log(0.1), size=(n_free, 1, 1))).astype(f32)
+    free = (c + size * rng.normal(size=(n_free, 3, 3))).astype(f32)
+    sliver = rng.random(n_free) < 0.2
+    t = rng.random((sliver.sum(), 1)).astype(f32)
+    free[sliver, 2] = (free[sliver, 0] * t + free[sliver, 1] * (1 - t)).astype(f32)
+    free[:, :, 2] = np.clip(free[:, :, 2], -0.95, 0.95)
+    pos = np.concatenate([grid, free]).reshape(-1, 3)
+    # random winding: about half the triangles face away
+    faces = np.arange(3 * n_tris, dtype=np.int32).reshape(n_tris, 3)
+    flip = rng.random(n_tris) < 0.5
+    faces[flip] = faces[flip][:, ::-1]
+    camera = Camera.create(
+        view=np.identity(4, dtype=f32),
+        projection=np.identity(4, dtype=f32),
+        viewport=Camera.viewport_matrix(np.zeros(2, np.int32), np.array((W, H)), 1.0),
+    )
+    kw = dict(init_buffers=True, zbuffer_default=1.0, colour_default=(0.1, 0.2, 0.3))
+    if shader is DepthShader:
+        return [build_call(camera, shader, faces, DepthExtraInput(position=pos), W, H, **kw)]
+    V = 3 * n_tris
+    ex = GouraudExtraInput(position=pos, colour=rng.uniform(0.0, 1.0, size=(V, 3)).astype(f32),
+                           normal=np.tile(np.array((0.0, 0.0, 1.0), f32), (V, 1)),
+                           light=LightSource(direction=np.array((0.0, 0.0, 1.0), f32)))
+    return [build_call(camera, shader, faces, ex, W, H, **kw)]

@@ -81,3 +81,15 @@ def test_config3_batch_against_oracle():
 
     scs, sm = workloads.config3(n_scenes=3, seed=11)
     _check(lambda: batch.plan_batch(scs, 84, 84, sm), "c3x3")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [(8, 8), (16, 16)], ids=str)
+def test_tiny_and_pixel_aligned_triangles(seed, tile):
+    """Sub-pixel triangles, slivers and vertices on exact sample positions:
+    the pixel-range and tile culling must keep every pixel the brute-force
+    oracle covers."""
+    from jaxrenderer_amd.shaders import GouraudShader
+
+    shader = DepthShader if seed % 2 == 0 else GouraudShader
+    _check(lambda: scenes.tiny_soup_call(seed, shader=shader), f"tiny{seed}", tile=tile)
