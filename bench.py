@@ -109,8 +109,12 @@ def main() -> None:
         t0 = time.time()
         tuned = br.autotune(use_cache=not args.no_autotune_cache)
         for tag, c in tuned.items():
-            log(f"autotune {tag}: tile {c['tile']} persistent={c['persistent']} "
+            log(f"autotune {tag}: tile {c['tile']} form {c['form']} "
                 f"{c['ms']:.3f} ms ({'cached' if c['cached'] else 'timed'})")
+            if not c["cached"]:
+                log("  candidates: " + ", ".join(
+                    f"{t['tile'][0]}x{t['tile'][1]}/{t['form']}={t['ms']:.3f}"
+                    for t in sorted(c["timings"], key=lambda t: t["ms"])))
         log(f"autotune took {time.time() - t0:.1f}s")
     pairs = br.calibrate()
     log(f"binned triangle-tile pairs: {pairs}")
@@ -297,7 +301,8 @@ def main() -> None:
             "width": W,
             "height": H,
             "shadow_map": [smspec.width, smspec.height],
-            "tile": {tag: {"tile": list(p.tile), "persistent": p.persistent}
+            "tile": {tag: {"tile": list(p.tile), "persistent": p.persistent,
+                           "tiles_per_group": p.tiles_per_group}
                      for tag, p in br.passes.items()},
             "autotuned": auto,
         },
@@ -373,7 +378,8 @@ def run_views(args) -> None:
                        "fwd_bwd_views_per_s": round(V / t_fb, 1),
                        "fwd_bwd_ms": round(t_fb * 1e3, 4),
                        "fwd_bwd_over_fwd": round(t_f / t_fb, 3), "grads_finite": finite,
-                       "tile": list(br.passes["main"].tile)}
+                       "tile": list(br.passes["main"].tile),
+                       "tiles_per_group": br.passes["main"].tiles_per_group}
         log(f"{name}: forward {t_f * 1e3:.3f} ms, forward+backward {t_fb * 1e3:.3f} ms "
             f"for {V} views")
     first = lines["GouraudTextureShader"]
@@ -411,7 +417,7 @@ def run_large(args) -> None:
     br = batch.BatchRenderer(calls, persistent=False)
     if args.tile == "auto":
         for tag, c in br.autotune(use_cache=not args.no_autotune_cache).items():
-            log(f"autotune {tag}: tile {c['tile']} persistent={c['persistent']}")
+            log(f"autotune {tag}: tile {c['tile']} form {c['form']}")
     pairs = br.calibrate()
     br.capture()
     t_f = _timed(br.graph.replay, args.steps, args.warmup)

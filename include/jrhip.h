@@ -150,7 +150,8 @@ typedef struct jr_frame {
   int32_t tile_w;         /* pixels per tile along width (x) */
   int32_t tile_h;         /* pixels per tile along height (y) */
   int32_t persistent;     /* 1: one persistent raster launch draining a work queue */
-  int32_t pad_i;
+  int32_t tiles_per_group; /* otherwise: consecutive tiles each workgroup rasterises
+                              (0 or 1: one) */
 } jr_frame;
 
 /* Gradient buffers for jr_backward (all optional, accumulated with +=). */

@@ -123,7 +123,7 @@ class JrFrame(ctypes.Structure):
         ("tile_w", c_i32),
         ("tile_h", c_i32),
         ("persistent", c_i32),
-        ("pad_i", c_i32),
+        ("tiles_per_group", c_i32),
     ]
 
 

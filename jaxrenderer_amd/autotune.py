@@ -2,9 +2,10 @@
 
 For every pass of a BatchRenderer (shadow, main) it times the candidate
 launch shapes of `jr_forward` on the live data: the screen-tile shape (which
-is also the workgroup shape: one lane per pixel of the tile), the persistent
-work-queue grid against one workgroup per tile, and, for each, the bin
-capacity sized from the binned triangle-tile pairs that shape produces.
+is also the workgroup shape: one lane per pixel of the tile), the launch form
+of the raster stage (a persistent work-queue grid, or a grid of workgroups
+taking 1, 2 or 4 consecutive tiles each), and, for each, the bin capacity
+sized from the binned triangle-tile pairs that shape produces.
 Every candidate must reproduce the first candidate's depth buffer bit for bit
 (the binning is exact, so the tile shape cannot change the image); a
 mismatch raises.
@@ -27,8 +28,10 @@ import torch
 from . import _raster
 
 TILES: Tuple[Tuple[int, int], ...] = ((8, 8), (4, 16), (16, 16), (8, 32))
-DEFAULT_CANDIDATES: Tuple[Tuple[Tuple[int, int], bool], ...] = tuple(
-    (t, p) for t in TILES for p in (True, False))
+# launch forms: 0 = persistent work queue, n > 0 = n tiles per workgroup
+FORMS: Tuple[int, ...] = (0, 1, 2, 4)
+DEFAULT_CANDIDATES: Tuple[Tuple[Tuple[int, int], int], ...] = tuple(
+    (t, m) for t in TILES for m in FORMS)
 SLACK = 1.25
 
 
@@ -68,53 +71,76 @@ def pass_key(tag: str, call, device: torch.device) -> str:
             f"scenes{call.n_scenes}|tris<={tris}")
 
 
-def _time_launches(p, reps: int) -> float:
-    """Milliseconds per launch of pass `p`, events on the launching stream."""
-    stream = torch.cuda.current_stream()
-    p.launch()
-    best = float("inf")
-    for _ in range(3):
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(stream)
+def _time_launches(p, reps: int, renderer=None) -> float:
+    """Milliseconds per launch of pass `p`: the sum over its stages of the
+    median stage time over `reps` launches, from the HIP events the library
+    records on the launching stream (medians per stage keep one slow launch
+    of one stage from deciding the choice).  With `renderer`, every launch is
+    a whole render() -- all passes in order, as a replay runs them -- and only
+    pass `p`'s stages are counted."""
+    import ctypes
+
+    import numpy as np
+
+    from . import _hip
+
+    L = _hip.lib()
+    passes = list(renderer.passes.values()) if renderer is not None else [p]
+    idx = next(i for i, q in enumerate(passes) if q is p)
+    run = renderer.render if renderer is not None else p.launch
+    run()
+    torch.cuda.synchronize()
+    L.jr_stage_times(None, 0)  # forget earlier records
+    L.jr_stage_timing(1)
+    try:
         for _ in range(reps):
-            p.launch()
-        b.record(stream)
-        b.synchronize()
-        best = min(best, a.elapsed_time(b) / reps)
-    return best
+            run()
+        torch.cuda.synchronize()
+    finally:
+        L.jr_stage_timing(0)
+    k, P = _hip.STAGES, len(passes)
+    buf = (ctypes.c_float * (k * reps * P))()
+    n = L.jr_stage_times(buf, reps * P)
+    st = np.array(buf[: k * n], dtype=np.float64).reshape(n // P, P, k)[:, idx, :]
+    return float(np.median(st, axis=0).sum())
 
 
-def tune_pass(p, candidates: Sequence[Tuple[Tuple[int, int], bool]] = DEFAULT_CANDIDATES,
-              reps: int = 5) -> Dict[str, object]:
-    """Times each (tile, persistent) candidate on pass `p` (a batch pass whose
-    inputs are ready, including any shadow map it reads) and leaves `p` set to
-    the fastest.  Returns the per-candidate timings and the choice."""
+def _apply(p, tile, form: int) -> None:
+    p.tile = tuple(tile)
+    p.persistent = form == 0
+    p.tiles_per_group = max(1, int(form))
+
+
+def tune_pass(p, candidates: Sequence[Tuple[Tuple[int, int], int]] = DEFAULT_CANDIDATES,
+              reps: int = 9, renderer=None) -> Dict[str, object]:
+    """Times each (tile, form) candidate on pass `p` (a batch pass whose
+    inputs are ready, including any shadow map it reads; with `renderer`,
+    inside whole renders) and leaves `p` set to the fastest.  Returns the
+    per-candidate timings and the choice."""
     timings: List[dict] = []
     ref: Optional[torch.Tensor] = None
-    for tile, persistent in candidates:
-        p.tile, p.persistent = tuple(tile), bool(persistent)
+    for tile, form in candidates:
+        _apply(p, tile, form)
         p.set_capacity(_raster.default_capacity(p.call.n_tris, p.n_tiles))
         p.launch()
         torch.cuda.synchronize()
         pairs = p.pairs()
         p.set_capacity(int(pairs * SLACK) + 1024)
-        ms = _time_launches(p, reps)
+        ms = _time_launches(p, reps, renderer)
         z = p.zbuffer.view(torch.int32)
         if ref is None:
             ref = z.clone()
         elif not torch.equal(ref, z):
-            raise RuntimeError(f"tile {tile} persistent={persistent} changed the depth buffer")
-        timings.append({"tile": list(tile), "persistent": bool(persistent), "ms": ms,
-                        "pairs": int(pairs)})
+            raise RuntimeError(f"tile {tile} form {form} changed the depth buffer")
+        timings.append({"tile": list(tile), "form": int(form), "ms": ms, "pairs": int(pairs)})
     best = min(timings, key=lambda t: t["ms"])
-    p.tile, p.persistent = tuple(best["tile"]), best["persistent"]
+    _apply(p, best["tile"], best["form"])
     p.set_capacity(int(best["pairs"] * SLACK) + 1024)
-    return {"tile": best["tile"], "persistent": best["persistent"], "ms": best["ms"],
-            "timings": timings}
+    return {"tile": best["tile"], "form": best["form"], "persistent": best["form"] == 0,
+            "tiles_per_group": max(1, best["form"]), "ms": best["ms"], "timings": timings}
 
 
-def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 5,
+def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 9,
          use_cache: bool = True, path: Optional[str] = None) -> Dict[str, dict]:
     """Picks tile shape, launch form and bin capacity for every pass of a
     BatchRenderer.  Cached choices are applied without timing (the capacity is
@@ -129,15 +155,14 @@ def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 5,
     for tag, p in renderer.passes.items():
         key = pass_key(tag, p.call, p.zbuffer.device)
         hit = cache.get(key)
-        if hit is not None and [tuple(hit["tile"]), hit["persistent"]] in \
-                [[tuple(t), q] for t, q in cands]:
-            got = tune_pass(p, [(tuple(hit["tile"]), hit["persistent"])], reps=1)
+        if hit is not None and "form" in hit and \
+                (tuple(hit["tile"]), hit["form"]) in [(tuple(t), m) for t, m in cands]:
+            got = tune_pass(p, [(tuple(hit["tile"]), hit["form"])], reps=1, renderer=renderer)
             got["cached"] = True
         else:
-            got = tune_pass(p, cands, reps=reps)
+            got = tune_pass(p, cands, reps=reps, renderer=renderer)
             got["cached"] = False
-            cache[key] = {"tile": got["tile"], "persistent": got["persistent"],
-                          "ms": got["ms"]}
+            cache[key] = {"tile": got["tile"], "form": got["form"], "ms": got["ms"]}
             dirty = True
         chosen[tag] = got
     if dirty and use_cache:
@@ -146,4 +171,5 @@ def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 5,
     return chosen
 
 
-__all__ = ["DEFAULT_CANDIDATES", "TILES", "cache_path", "pass_key", "tune", "tune_pass"]
+__all__ = ["DEFAULT_CANDIDATES", "FORMS", "TILES", "cache_path", "pass_key", "tune",
+           "tune_pass"]

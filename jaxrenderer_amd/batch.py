@@ -160,6 +160,7 @@ class _Pass:
         self.call = call
         self.tile = tile
         self.persistent = persistent
+        self.tiles_per_group = 1
         self.inputs = {k: _plan.to_device(v, dev, k) for k, v in call.inputs.items()}
         self.scenes = _raster.scenes_to_device(call.scenes, dev)
         B, W, H = call.n_scenes, call.width, call.height
@@ -193,7 +194,8 @@ class _Pass:
             display=self.display, shadow_shape=c.shadow_shape, init_buffers=c.init_buffers,
             zbuffer_default=c.zbuffer_default, colour_default=c.colour_default,
             apply_offset=c.apply_offset, depth_offset=c.depth_offset, tile=self.tile,
-            persistent=self.persistent, capacity=self.capacity, workspace=self.workspace)
+            persistent=self.persistent, tiles_per_group=self.tiles_per_group,
+            capacity=self.capacity, workspace=self.workspace)
 
     def pairs(self) -> int:
         c = self.call

@@ -87,24 +87,31 @@ JR_DEV float max_grad(float a, float b) {
   return a > b ? 1.0f : (a == b ? 0.5f : 0.0f);
 }
 
-JR_DEV void add_tex(float* __restrict__ d, int64_t base, int32_t th, int32_t tc, int32_t u,
-                    int32_t v, const float g[3]) {
-  if (!d) return;
-  const int64_t o = base + ((int64_t)u * th + v) * tc;
-  if (tc > 1) {
-    atomicAdd(d + o, g[0]);
-    atomicAdd(d + o + 1, g[1]);
-    atomicAdd(d + o + 2, g[2]);
-  } else {
-    atomicAdd(d + o, (g[0] + g[1]) + g[2]);
-  }
-}
-
 struct PixGrad {
   float db[3];
   float dvar[3][3];
   float sg[S_COUNT];
+  // the texel and specular-map entry this pixel's gradient lands on (-1:
+  // none); scattered after the per-pixel work, with the wavefront converged
+  int64_t tex_at, spec_at;
+  int32_t tex_n;
+  float tex_g[3], spec_g;
 };
+
+// A texel lookup transposes to an add at the same (already clamped) index.
+JR_DEV void add_tex(PixGrad& pg, int64_t base, int32_t th, int32_t tc, int32_t u, int32_t v,
+                    const float g[3]) {
+  pg.tex_at = base + ((int64_t)u * th + v) * tc;
+  if (tc > 1) {
+    pg.tex_n = 3;
+    pg.tex_g[0] = g[0];
+    pg.tex_g[1] = g[1];
+    pg.tex_g[2] = g[2];
+  } else {
+    pg.tex_n = 1;
+    pg.tex_g[0] = (g[0] + g[1]) + g[2];
+  }
+}
 
 template <int SHADER>
 struct ShadeBwd;
@@ -159,7 +166,7 @@ struct ShadeBwd<JR_SHADER_GOURAUD_TEXTURE> {
       dt[q] = dcol[q] * light[q];
       dl[q] = dcol[q] * tex[q];
     }
-    add_tex(gr.d_textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
     for (int k = 0; k < 3; ++k) {
       g.db[k] = dot3(dl, lc[k]);
       for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dl[q];
@@ -204,7 +211,7 @@ struct ShadeBwd<JR_SHADER_PHONG> {
       g.sg[S_LIGHT_COLOUR + q] += dlc[q] * d;
       dd += dlc[q] * sc.light_colour[q];
     }
-    add_tex(gr.d_textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
     float dnn[3], dln[3], dn[3], dL[3];
     for (int q = 0; q < 3; ++q) {
       dnn[q] = dd * ln[q];
@@ -299,10 +306,11 @@ struct ReflectionBwd {
       ddiff += dk * sc.diffuse[q];
       dspec += dk * sc.specular[q];
     }
-    add_tex(gr.d_textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
     if (gr.d_specular_maps) {
       const float lg = logf(m == 0.0f ? 1.0f : m);
-      atomicAdd(gr.d_specular_maps + spec_at, dspec * lg * spec);
+      g.spec_at = spec_at;
+      g.spec_g = dspec * lg * spec;
     }
     const float dm = dspec * (S == 0.0f ? 0.0f : S * powf(m, S - 1.0f));
     const float dr[3] = {0.0f, 0.0f, dm * max_grad(rh[2], 0.0f)};
@@ -341,6 +349,35 @@ JR_DEV float wave_sum(float v) {
   return v;
 }
 
+// d[at + c] += v[c] (c < n) for every lane with at >= 0; called with all 64
+// lanes of the wavefront active.  Lanes that hit the same entry are summed
+// first (a wavefront reduction per distinct entry, for up to 8 entries), so a
+// magnified texture -- many pixels on one texel -- costs one global atomic per
+// texel and tile instead of one per pixel; the remaining lanes add directly.
+JR_DEV void scatter_add(float* __restrict__ d, int64_t at, int32_t n, const float* v) {
+  if (!d) return;
+  const int lane = threadIdx.x & 63;
+  bool want = at >= 0;
+  uint64_t pending = __ballot(want);
+  for (int it = 0; pending && it < 8; ++it) {
+    const int l = __ffsll((unsigned long long)pending) - 1;
+    const int32_t lo = __shfl((int32_t)(uint32_t)(uint64_t)at, l);
+    const int32_t hi = __shfl((int32_t)(uint32_t)((uint64_t)at >> 32), l);
+    const int64_t k = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    const int32_t nk = __shfl(n, l);
+    const bool mine = want && at == k;
+    pending &= ~__ballot(mine);
+    for (int c = 0; c < 3; ++c) {
+      if (c >= nk) break;
+      const float sum = wave_sum(mine ? v[c] : 0.0f);
+      if (lane == l) atomicAdd(d + k + c, sum);
+    }
+    if (mine) want = false;
+  }
+  if (want)
+    for (int c = 0; c < n; ++c) atomicAdd(d + at + c, v[c]);
+}
+
 struct BwdArgs {
   jr_inputs in;
   const jr_scene* scenes;
@@ -369,6 +406,11 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
 
   PixGrad g;
   for (int q = 0; q < S_COUNT; ++q) g.sg[q] = 0.0f;
+  g.tex_at = -1;
+  g.spec_at = -1;
+  g.tex_n = 0;
+  g.spec_g = 0.0f;
+  for (int q = 0; q < 3; ++q) g.tex_g[q] = 0.0f;
   float vals[kAcc];
   for (int q = 0; q < kAcc; ++q) vals[q] = 0.0f;
   if (t >= 0) {
@@ -429,6 +471,9 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
     for (int k = 0; k < 3; ++k)
       for (int q = 0; q < 3; ++q) vals[A_VAR + 3 * k + q] = g.dvar[k][q];
   }
+
+  scatter_add(a.gr.d_textures, g.tex_at, g.tex_n, g.tex_g);
+  scatter_add(a.gr.d_specular_maps, g.spec_at, 1, &g.spec_g);
 
   // group the lanes by triangle: leader = lowest lane with the same id
   int leader = lane;

@@ -522,6 +522,7 @@ struct RasterArgs {
   int32_t* work_counter;
   int32_t chunk;  // tiles claimed per work-queue step (persistent form)
   int32_t xcd_scenes;  // one-group-per-tile form: scenes kept on one XCD each
+  int32_t tiles_per_group;  // tiles each group of the non-persistent form takes
   int32_t init_buffers, apply_offset;
   float zdef, cdef0, cdef1, cdef2, depth_offset;
 };
@@ -690,15 +691,27 @@ template <int SHADER, int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void k_raster(RasterArgs a) {
   __shared__ float4 lds_rec[3 * TW * TH];
   __shared__ int32_t lds_id[TW * TH];
-  int64_t gt = blockIdx.x;
+  const int64_t b = blockIdx.x;
   if (a.xcd_scenes) {
-    const int64_t b = blockIdx.x;
-    const int64_t k = b / kXcds;
-    const int64_t s = (k / a.f.T) * kXcds + (b - k * kXcds);
-    if (s >= a.f.n_scenes) return;
-    gt = s * a.f.T + (k - (k / a.f.T) * a.f.T);
+    // the XCD's own list: tiles of scenes x, x + 8, x + 16, ... in order
+    const int32_t x = (int32_t)(b % kXcds);
+    const int64_t first = (b / kXcds) * a.tiles_per_group;
+    const int64_t n_x = (int64_t)((a.f.n_scenes - x + kXcds - 1) / kXcds) * a.f.T;
+    for (int32_t q = 0; q < a.tiles_per_group; ++q) {
+      const int64_t k = first + q;
+      if (k >= n_x) break;
+      const int64_t sl = k / a.f.T;
+      raster_tile<SHADER, TW, TH>(a, (sl * kXcds + x) * a.f.T + (k - sl * a.f.T), lds_rec,
+                                  lds_id);
+    }
+  } else {
+    const int64_t n_items = (int64_t)a.f.n_scenes * a.f.T;
+    for (int32_t q = 0; q < a.tiles_per_group; ++q) {
+      const int64_t gt = b * a.tiles_per_group + q;
+      if (gt >= n_items) break;
+      raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
+    }
   }
-  raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
 }
 
 // Persistent form: a fixed grid drains the (scene, tile) queue.  Every group
@@ -804,6 +817,8 @@ int32_t check_frame(const jr_frame* fr) {
   if (!ok_tile) return fail(JR_E_INVALID, "unsupported tile shape");
   if (fr->bin_capacity < 0 || fr->bin_capacity > 0x7FFFFFFFLL)
     return fail(JR_E_INVALID, "bin_capacity out of range");
+  if (fr->tiles_per_group < 0 || fr->tiles_per_group > 64)
+    return fail(JR_E_INVALID, "tiles_per_group must be in [0, 64]");
   return JR_OK;
 }
 
@@ -824,10 +839,14 @@ void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
                        dim3(TW * TH), 0, st, b, n_items);
   } else {
     RasterArgs b = a;
-    int64_t grid = n_items;
+    const int64_t tpg = std::max(1, a.tiles_per_group);
+    b.tiles_per_group = (int32_t)tpg;
+    int64_t grid = (n_items + tpg - 1) / tpg;
     b.xcd_scenes = a.f.n_scenes >= 8 * kXcds;
-    if (b.xcd_scenes)
-      grid = (int64_t)kXcds * ((a.f.n_scenes + kXcds - 1) / kXcds) * a.f.T;
+    if (b.xcd_scenes) {
+      const int64_t per_xcd = (int64_t)((a.f.n_scenes + kXcds - 1) / kXcds) * a.f.T;
+      grid = kXcds * ((per_xcd + tpg - 1) / tpg);
+    }
     hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)grid),
                        dim3(TW * TH), 0, st, b);
   }
@@ -999,6 +1018,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.work_counter = misc + 1;
   a.chunk = 1;
   a.xcd_scenes = 0;
+  a.tiles_per_group = fr->tiles_per_group;
   a.init_buffers = fr->init_buffers;
   a.apply_offset = fr->apply_offset;
   a.zdef = fr->zbuffer_default;

@@ -213,6 +213,8 @@ def config4(seed: int = 0, n_views: int = 64, size: int = 256, tex_size: int = 5
         noise[1:-1] = (noise[1:-1] + noise[:-2] + noise[2:]) / 3.0
     radial = 1.0 + 0.05 * noise / max(noise.std(), 1e-6)
     v, n, uv, f = uv_sphere(n_lat, n_lon, 1.0, 0.0, radial)
+    # gouraud_texture and phong index texture[floor(uv) % shape]: texel units
+    uv = (uv * f32(tex_size)).astype(f32)
     tex = rng.random((tex_size, tex_size, 3)).astype(f32)
     colours = rng.uniform(0.3, 1.0, size=(len(v), 3)).astype(f32)
     if perturbed:

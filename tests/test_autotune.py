@@ -15,7 +15,7 @@ from jaxrenderer_amd import autotune, batch, workloads
 def test_cache_roundtrip(tmp_path):
     path = str(tmp_path / "sub" / "tune.json")
     assert autotune._load(path) == {}
-    autotune._store(path, {"k": {"tile": [8, 8], "persistent": True, "ms": 1.0}})
+    autotune._store(path, {"k": {"tile": [8, 8], "form": 0, "ms": 1.0}})
     assert autotune._load(path)["k"]["tile"] == [8, 8]
     with open(path, "w") as f:
         f.write("not json")
@@ -24,9 +24,9 @@ def test_cache_roundtrip(tmp_path):
 
 def test_candidates_are_supported_tiles():
     assert (8, 8) in autotune.TILES
-    for (tw, th), persistent in autotune.DEFAULT_CANDIDATES:
+    for (tw, th), form in autotune.DEFAULT_CANDIDATES:
         assert (tw * th) % 64 == 0  # whole wavefronts per workgroup
-        assert isinstance(persistent, bool)
+        assert form in autotune.FORMS and 0 <= form <= 64
 
 
 def test_cache_path_env(monkeypatch, tmp_path):
@@ -50,7 +50,8 @@ def test_autotune_small_batch(tmp_path):
     for tag, c in got.items():
         assert not c["cached"]
         assert len(c["timings"]) == len(autotune.DEFAULT_CANDIDATES)
-        assert (tuple(c["tile"]), c["persistent"]) in autotune.DEFAULT_CANDIDATES
+        assert (tuple(c["tile"]), c["form"]) in autotune.DEFAULT_CANDIDATES
+        assert br.passes[tag].persistent == (c["form"] == 0)
         assert br.passes[tag].tile == tuple(c["tile"])
     out = br.replay()
     torch.cuda.synchronize()

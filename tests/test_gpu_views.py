@@ -7,10 +7,11 @@ batched forward/backward over all views sharing the mesh and texture.
   (1e-5 relative: only the order of the atomic sums differs); per-view
   camera gradients equal the single-view ones.
 * Central differences (h = 1e-3) of the L2 image loss for randomly chosen
-  texel values, per-vertex colours, light direction and vertex coordinates.
-  Vertex coordinates whose +-h moves any primitive id are excluded (the
-  gradient is that of the piecewise-constant visibility, as under jax.grad)
-  and counted; the rest must agree with the analytic gradient.
+  texel values, per-vertex colours, light direction and (untextured shader)
+  vertex coordinates.  Parameters whose +-h moves any primitive id are
+  excluded (the gradient is that of the piecewise-constant visibility, as
+  under jax.grad) and counted; the rest must agree with the analytic
+  gradient.
 """
 import numpy as np
 import pytest
@@ -108,7 +109,13 @@ def test_views_finite_differences(shader):
     cams_t, ex = _leaves(cams, extra, dev)
     zb, cv = render_views_differentiable(cams_t, shader, faces, ex, W, H)
     loss = ((cv - torch.as_tensor(target, device=dev).float()) ** 2).sum()
-    names = [k for k in ("texture", "colour", "position") if hasattr(ex, k)]
+    # vertex coordinates only where no texture is read: moving a vertex also
+    # moves the interpolated uv, and where that crosses a texel edge the loss
+    # jumps (floor has no derivative, as under jax.grad), which central
+    # differences would pick up
+    names = [k for k in ("texture", "colour") if hasattr(ex, k)]
+    if not hasattr(ex, "texture"):
+        names.append("position")
     grads = dict(zip(names + ["light_direction"],
                      torch.autograd.grad(loss, [getattr(ex, k) for k in names]
                                          + [ex.light.direction])))
@@ -142,4 +149,4 @@ def test_views_finite_differences(shader):
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[k]) <= 2e-2 * max(1.0, abs(fd)), f"{name}[{k}]: fd {fd} vs {g[k]}"
             checked += 1
-    assert checked >= 30, (checked, excluded)
+    assert checked >= 25, (checked, excluded)
