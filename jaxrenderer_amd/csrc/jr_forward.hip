@@ -687,30 +687,45 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
 // 8 * ceil(n_scenes / 8) * T groups; the padding groups exit at once.
 constexpr int kXcds = 8;
 
+// (scene, tile) item of the k-th tile of XCD x's list: the tiles of scenes
+// x, x + 8, x + 16, ... in order (-1 past its end).
+JR_DEV int64_t xcd_item(const RasterArgs& a, int32_t x, int64_t k) {
+  const int64_t n_x = (int64_t)((a.f.n_scenes - x + kXcds - 1) / kXcds) * a.f.T;
+  if (k >= n_x) return -1;
+  const int64_t sl = k / a.f.T;
+  return (sl * kXcds + x) * a.f.T + (k - sl * a.f.T);
+}
+
 template <int SHADER, int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void k_raster(RasterArgs a) {
   __shared__ float4 lds_rec[3 * TW * TH];
   __shared__ int32_t lds_id[TW * TH];
-  const int64_t b = blockIdx.x;
+  int64_t gt = blockIdx.x;
   if (a.xcd_scenes) {
-    // the XCD's own list: tiles of scenes x, x + 8, x + 16, ... in order
-    const int32_t x = (int32_t)(b % kXcds);
-    const int64_t first = (b / kXcds) * a.tiles_per_group;
-    const int64_t n_x = (int64_t)((a.f.n_scenes - x + kXcds - 1) / kXcds) * a.f.T;
-    for (int32_t q = 0; q < a.tiles_per_group; ++q) {
-      const int64_t k = first + q;
-      if (k >= n_x) break;
-      const int64_t sl = k / a.f.T;
-      raster_tile<SHADER, TW, TH>(a, (sl * kXcds + x) * a.f.T + (k - sl * a.f.T), lds_rec,
-                                  lds_id);
-    }
-  } else {
-    const int64_t n_items = (int64_t)a.f.n_scenes * a.f.T;
-    for (int32_t q = 0; q < a.tiles_per_group; ++q) {
-      const int64_t gt = b * a.tiles_per_group + q;
+    gt = xcd_item(a, (int32_t)(blockIdx.x % kXcds), blockIdx.x / kXcds);
+    if (gt < 0) return;
+  }
+  raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
+}
+
+// Several consecutive tiles per workgroup (a separate kernel: the loop raises
+// the register count of the single-tile form).
+template <int SHADER, int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void k_raster_multi(RasterArgs a) {
+  __shared__ float4 lds_rec[3 * TW * TH];
+  __shared__ int32_t lds_id[TW * TH];
+  const int64_t b = blockIdx.x;
+  const int64_t n_items = (int64_t)a.f.n_scenes * a.f.T;
+  for (int32_t q = 0; q < a.tiles_per_group; ++q) {
+    int64_t gt;
+    if (a.xcd_scenes) {
+      gt = xcd_item(a, (int32_t)(b % kXcds), (b / kXcds) * a.tiles_per_group + q);
+      if (gt < 0) break;
+    } else {
+      gt = b * a.tiles_per_group + q;
       if (gt >= n_items) break;
-      raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
     }
+    raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
   }
 }
 
@@ -847,8 +862,12 @@ void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
       const int64_t per_xcd = (int64_t)((a.f.n_scenes + kXcds - 1) / kXcds) * a.f.T;
       grid = kXcds * ((per_xcd + tpg - 1) / tpg);
     }
-    hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)grid),
-                       dim3(TW * TH), 0, st, b);
+    if (tpg == 1)
+      hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)grid),
+                         dim3(TW * TH), 0, st, b);
+    else
+      hipLaunchKernelGGL((k_raster_multi<SHADER, TW, TH>), dim3((unsigned)grid),
+                         dim3(TW * TH), 0, st, b);
   }
 }
 
