@@ -109,11 +109,11 @@ def main() -> None:
         t0 = time.time()
         tuned = br.autotune(use_cache=not args.no_autotune_cache)
         for tag, c in tuned.items():
-            log(f"autotune {tag}: tile {c['tile']} form {c['form']} "
+            log(f"autotune {tag}: tile {c['tile']} form {c['form']} x{c['ppl']} "
                 f"{c['ms']:.3f} ms ({'cached' if c['cached'] else 'timed'})")
             if not c["cached"]:
                 log("  candidates: " + ", ".join(
-                    f"{t['tile'][0]}x{t['tile'][1]}/{t['form']}={t['ms']:.3f}"
+                    f"{t['tile'][0]}x{t['tile'][1]}/{t['form']}/x{t['ppl']}={t['ms']:.3f}"
                     for t in sorted(c["timings"], key=lambda t: t["ms"])))
         log(f"autotune took {time.time() - t0:.1f}s")
     pairs = br.calibrate()
@@ -302,7 +302,8 @@ def main() -> None:
             "height": H,
             "shadow_map": [smspec.width, smspec.height],
             "tile": {tag: {"tile": list(p.tile), "persistent": p.persistent,
-                           "tiles_per_group": p.tiles_per_group}
+                           "tiles_per_group": p.tiles_per_group,
+                           "pixels_per_lane": p.pixels_per_lane}
                      for tag, p in br.passes.items()},
             "autotuned": auto,
         },
@@ -417,7 +418,7 @@ def run_large(args) -> None:
     br = batch.BatchRenderer(calls, persistent=False)
     if args.tile == "auto":
         for tag, c in br.autotune(use_cache=not args.no_autotune_cache).items():
-            log(f"autotune {tag}: tile {c['tile']} form {c['form']}")
+            log(f"autotune {tag}: tile {c['tile']} form {c['form']} x{c['ppl']}")
     pairs = br.calibrate()
     br.capture()
     t_f = _timed(br.graph.replay, args.steps, args.warmup)

@@ -152,6 +152,9 @@ typedef struct jr_frame {
   int32_t persistent;     /* 1: one persistent raster launch draining a work queue */
   int32_t tiles_per_group; /* otherwise: consecutive tiles each workgroup rasterises
                               (0 or 1: one) */
+  int32_t pixels_per_lane; /* pixels per raster thread: 0/1, or 4 with 16x16 or 8x32
+                              tiles (one wavefront per tile) */
+  int32_t pad_i;
 } jr_frame;
 
 /* Gradient buffers for jr_backward (all optional, accumulated with +=). */

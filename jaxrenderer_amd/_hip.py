@@ -124,6 +124,8 @@ class JrFrame(ctypes.Structure):
         ("tile_h", c_i32),
         ("persistent", c_i32),
         ("tiles_per_group", c_i32),
+        ("pixels_per_lane", c_i32),
+        ("pad_i", c_i32),
     ]
 
 

@@ -62,7 +62,7 @@ def to_device(x, dev: torch.device, name: str) -> Optional[torch.Tensor]:
 def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         canvas: Optional[torch.Tensor] = None, prim_id: bool = False, bary: bool = False,
         display: bool = False, tile=_raster.DEFAULT_TILE, persistent: bool = False,
-        tiles_per_group: int = 1,
+        tiles_per_group: int = 1, pixels_per_lane: int = 1,
         capacity: Optional[int] = None) -> Dict[str, torch.Tensor]:
     """Executes `call` on the GPU.  zbuffer/canvas given here are updated in
     place (merge semantics when call.init_buffers is False)."""
@@ -98,7 +98,7 @@ def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         init_buffers=call.init_buffers, zbuffer_default=call.zbuffer_default,
         colour_default=call.colour_default, apply_offset=call.apply_offset,
         depth_offset=call.depth_offset, tile=tile, persistent=persistent,
-        tiles_per_group=tiles_per_group, capacity=capacity,
+        tiles_per_group=tiles_per_group, pixels_per_lane=pixels_per_lane, capacity=capacity,
     )
     return out
 

@@ -161,7 +161,7 @@ def default_capacity(n_tris: int, n_tiles: int) -> int:
 def make_frame(*, width, height, n_scenes, shader, init_buffers=False, apply_offset=False,
                zbuffer_default=1.0, colour_default=(1.0, 1.0, 1.0), depth_offset=0.0,
                tile=DEFAULT_TILE, persistent=False, capacity=0,
-               tiles_per_group: int = 1) -> _hip.JrFrame:
+               tiles_per_group: int = 1, pixels_per_lane: int = 1) -> _hip.JrFrame:
     fr = _hip.JrFrame()
     fr.bin_capacity = int(capacity)
     fr.width, fr.height, fr.n_scenes, fr.shader = int(width), int(height), int(n_scenes), int(shader)
@@ -176,6 +176,7 @@ def make_frame(*, width, height, n_scenes, shader, init_buffers=False, apply_off
     fr.tile_w, fr.tile_h = int(tile[0]), int(tile[1])
     fr.persistent = int(bool(persistent))
     fr.tiles_per_group = int(tiles_per_group)
+    fr.pixels_per_lane = int(pixels_per_lane)
     return fr
 
 
@@ -207,6 +208,7 @@ def forward(
     tile=DEFAULT_TILE,
     persistent: bool = False,
     tiles_per_group: int = 1,
+    pixels_per_lane: int = 1,
     capacity: Optional[int] = None,
     stream: Optional[torch.cuda.Stream] = None,
     workspace: Optional[torch.Tensor] = None,
@@ -223,7 +225,7 @@ def forward(
                     init_buffers=init_buffers, apply_offset=apply_offset,
                     zbuffer_default=zbuffer_default, colour_default=colour_default,
                     depth_offset=depth_offset, tile=tile, persistent=persistent, capacity=cap,
-                    tiles_per_group=tiles_per_group)
+                    tiles_per_group=tiles_per_group, pixels_per_lane=pixels_per_lane)
     jin = _hip.JrInputs()
     for name, dt in _INPUT_FIELDS:
         setattr(jin, name, _ptr(inputs.get(name), name, dt, dev))

@@ -2,10 +2,11 @@
 
 For every pass of a BatchRenderer (shadow, main) it times the candidate
 launch shapes of `jr_forward` on the live data: the screen-tile shape (which
-is also the workgroup shape: one lane per pixel of the tile), the launch form
-of the raster stage (a persistent work-queue grid, or a grid of workgroups
-taking 1, 2 or 4 consecutive tiles each), and, for each, the bin capacity
-sized from the binned triangle-tile pairs that shape produces.
+also sets the workgroup: one lane per pixel, or per 4 pixels for the
+256-pixel tiles), the launch form of the raster stage (a persistent
+work-queue grid, or a grid of workgroups taking 1, 2 or 4 consecutive tiles
+each), and, for each, the bin capacity sized from the binned triangle-tile
+pairs that shape produces.
 Every candidate must reproduce the first candidate's depth buffer bit for bit
 (the binning is exact, so the tile shape cannot change the image); a
 mismatch raises.
@@ -30,8 +31,11 @@ from . import _raster
 TILES: Tuple[Tuple[int, int], ...] = ((8, 8), (4, 16), (16, 16), (8, 32))
 # launch forms: 0 = persistent work queue, n > 0 = n tiles per workgroup
 FORMS: Tuple[int, ...] = (0, 1, 2, 4)
-DEFAULT_CANDIDATES: Tuple[Tuple[Tuple[int, int], int], ...] = tuple(
-    (t, m) for t in TILES for m in FORMS)
+# pixels per raster thread: 4 only for the 256-pixel tiles (one wavefront each)
+PIXELS_PER_LANE: Tuple[int, ...] = (1, 4)
+DEFAULT_CANDIDATES: Tuple[Tuple[Tuple[int, int], int, int], ...] = tuple(
+    (t, m, ppl) for t in TILES for m in FORMS for ppl in PIXELS_PER_LANE
+    if ppl == 1 or t[0] * t[1] == 256)
 SLACK = 1.25
 
 
@@ -105,22 +109,23 @@ def _time_launches(p, reps: int, renderer=None) -> float:
     return float(np.median(st, axis=0).sum())
 
 
-def _apply(p, tile, form: int) -> None:
+def _apply(p, tile, form: int, ppl: int) -> None:
     p.tile = tuple(tile)
     p.persistent = form == 0
     p.tiles_per_group = max(1, int(form))
+    p.pixels_per_lane = int(ppl)
 
 
-def tune_pass(p, candidates: Sequence[Tuple[Tuple[int, int], int]] = DEFAULT_CANDIDATES,
-              reps: int = 9, renderer=None) -> Dict[str, object]:
-    """Times each (tile, form) candidate on pass `p` (a batch pass whose
-    inputs are ready, including any shadow map it reads; with `renderer`,
-    inside whole renders) and leaves `p` set to the fastest.  Returns the
-    per-candidate timings and the choice."""
+def tune_pass(p, candidates: Sequence = DEFAULT_CANDIDATES, reps: int = 9,
+              renderer=None) -> Dict[str, object]:
+    """Times each (tile, form, pixels per lane) candidate on pass `p` (a
+    batch pass whose inputs are ready, including any shadow map it reads; with
+    `renderer`, inside whole renders) and leaves `p` set to the fastest.
+    Returns the per-candidate timings and the choice."""
     timings: List[dict] = []
     ref: Optional[torch.Tensor] = None
-    for tile, form in candidates:
-        _apply(p, tile, form)
+    for tile, form, ppl in candidates:
+        _apply(p, tile, form, ppl)
         p.set_capacity(_raster.default_capacity(p.call.n_tris, p.n_tiles))
         p.launch()
         torch.cuda.synchronize()
@@ -131,13 +136,15 @@ def tune_pass(p, candidates: Sequence[Tuple[Tuple[int, int], int]] = DEFAULT_CAN
         if ref is None:
             ref = z.clone()
         elif not torch.equal(ref, z):
-            raise RuntimeError(f"tile {tile} form {form} changed the depth buffer")
-        timings.append({"tile": list(tile), "form": int(form), "ms": ms, "pairs": int(pairs)})
+            raise RuntimeError(f"tile {tile} form {form} x{ppl} changed the depth buffer")
+        timings.append({"tile": list(tile), "form": int(form), "ppl": int(ppl), "ms": ms,
+                        "pairs": int(pairs)})
     best = min(timings, key=lambda t: t["ms"])
-    _apply(p, best["tile"], best["form"])
+    _apply(p, best["tile"], best["form"], best["ppl"])
     p.set_capacity(int(best["pairs"] * SLACK) + 1024)
-    return {"tile": best["tile"], "form": best["form"], "persistent": best["form"] == 0,
-            "tiles_per_group": max(1, best["form"]), "ms": best["ms"], "timings": timings}
+    return {"tile": best["tile"], "form": best["form"], "ppl": best["ppl"],
+            "persistent": best["form"] == 0, "tiles_per_group": max(1, best["form"]),
+            "ms": best["ms"], "timings": timings}
 
 
 def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 9,
@@ -155,14 +162,16 @@ def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 9,
     for tag, p in renderer.passes.items():
         key = pass_key(tag, p.call, p.zbuffer.device)
         hit = cache.get(key)
-        if hit is not None and "form" in hit and \
-                (tuple(hit["tile"]), hit["form"]) in [(tuple(t), m) for t, m in cands]:
-            got = tune_pass(p, [(tuple(hit["tile"]), hit["form"])], reps=1, renderer=renderer)
+        entry = None if hit is None or "ppl" not in hit else (
+            tuple(hit["tile"]), hit["form"], hit["ppl"])
+        if entry is not None and entry in [(tuple(t), m, q) for t, m, q in cands]:
+            got = tune_pass(p, [entry], reps=1, renderer=renderer)
             got["cached"] = True
         else:
             got = tune_pass(p, cands, reps=reps, renderer=renderer)
             got["cached"] = False
-            cache[key] = {"tile": got["tile"], "form": got["form"], "ms": got["ms"]}
+            cache[key] = {"tile": got["tile"], "form": got["form"], "ppl": got["ppl"],
+                          "ms": got["ms"]}
             dirty = True
         chosen[tag] = got
     if dirty and use_cache:
@@ -171,5 +180,5 @@ def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 9,
     return chosen
 
 
-__all__ = ["DEFAULT_CANDIDATES", "FORMS", "TILES", "cache_path", "pass_key", "tune",
-           "tune_pass"]
+__all__ = ["DEFAULT_CANDIDATES", "FORMS", "PIXELS_PER_LANE", "TILES", "cache_path",
+           "pass_key", "tune", "tune_pass"]

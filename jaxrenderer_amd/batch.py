@@ -161,6 +161,7 @@ class _Pass:
         self.tile = tile
         self.persistent = persistent
         self.tiles_per_group = 1
+        self.pixels_per_lane = 1
         self.inputs = {k: _plan.to_device(v, dev, k) for k, v in call.inputs.items()}
         self.scenes = _raster.scenes_to_device(call.scenes, dev)
         B, W, H = call.n_scenes, call.width, call.height
@@ -195,6 +196,7 @@ class _Pass:
             zbuffer_default=c.zbuffer_default, colour_default=c.colour_default,
             apply_offset=c.apply_offset, depth_offset=c.depth_offset, tile=self.tile,
             persistent=self.persistent, tiles_per_group=self.tiles_per_group,
+            pixels_per_lane=self.pixels_per_lane,
             capacity=self.capacity, workspace=self.workspace)
 
     def pairs(self) -> int:

@@ -523,73 +523,19 @@ struct RasterArgs {
   int32_t chunk;  // tiles claimed per work-queue step (persistent form)
   int32_t xcd_scenes;  // one-group-per-tile form: scenes kept on one XCD each
   int32_t tiles_per_group;  // tiles each group of the non-persistent form takes
+  int32_t pixels_per_lane;  // 1, or 4 (16x16 and 8x32 tiles): pixels per thread
   int32_t init_buffers, apply_offset;
   float zdef, cdef0, cdef1, cdef2, depth_offset;
 };
 
-template <int SHADER, int TW, int TH>
-JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
-                        int32_t* lds_id) {
-  constexpr int NT = TW * TH;
+// Chooser fallback, shading, merge and stores of pixel (i, j) of scene s
+// whose running minimum over all candidates is (best_key, best_id).
+template <int SHADER>
+JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int32_t i,
+                         int32_t j, float x, float y, uint32_t best_key, int32_t best_id) {
   const Frame& f = a.f;
-  const int32_t s = (int32_t)(gt / f.T);
-  const int32_t tile = (int32_t)(gt - (int64_t)s * f.T);
-  const int32_t tx = tile / f.tiles_y, ty = tile - tx * f.tiles_y;
-  const int tid = threadIdx.x;
-  const int32_t i = tx * TW + tid / TH;
-  const int32_t j = ty * TH + tid % TH;
-  const bool inside = (i < f.W) && (j < f.H);
-  const jr_scene& sc = a.scenes[s];
   const float* vp = sc.viewport;
-  const float x = pix_x(i, vp), y = pix_y(j, vp);
   const float vp22 = vp[10], vp23 = vp[11];
-
-  const int32_t cnt = a.counts[gt];
-  const int64_t start = a.offsets[gt];
-  const bool overflow = start + cnt > f.capacity;
-  // overflowing tile lists fall back to every triangle of the scene
-  const int32_t n_items = overflow ? (sc.tri_end - sc.tri_begin) : cnt;
-
-  // start from the small triangles already resolved by k_setup
-  uint32_t best_key = 0xFFFFFFFFu;
-  int32_t best_id = 0x7FFFFFFF;
-  if (inside) {
-    const uint64_t k0 = a.keys[(int64_t)s * f.W * f.H + (int64_t)i * f.H + j];
-    if (k0 != ~0ull) {
-      best_key = (uint32_t)(k0 >> 32);
-      best_id = (int32_t)(uint32_t)k0;
-    }
-  }
-  for (int32_t base = 0; base < n_items; base += NT) {
-    const int32_t n = min(NT, n_items - base);
-    __syncthreads();
-    if (tid < n) {
-      const int32_t t = overflow ? (sc.tri_begin + base + tid) : a.bins[start + base + tid];
-      const float4 r0 = a.rec[4 * (int64_t)t], r1 = a.rec[4 * (int64_t)t + 1],
-                   r2 = a.rec[4 * (int64_t)t + 2];
-      const uint32_t fl = __float_as_uint(a.rec[4 * (int64_t)t + 3].y);
-      lds_rec[3 * tid + 0] = r0;
-      lds_rec[3 * tid + 1] = r1;
-      lds_rec[3 * tid + 2] = r2;
-      // triangles that are not kept or not front-facing never win the chooser
-      lds_id[tid] = ((fl & (kKeep | kFront)) == (kKeep | kFront)) ? t : -1;
-    }
-    __syncthreads();
-    for (int q = 0; q < n; ++q) {
-      const int32_t t = lds_id[q];
-      const float4 r0 = lds_rec[3 * q + 0], r1 = lds_rec[3 * q + 1],
-                   r2 = lds_rec[3 * q + 2];
-      const float e0 = edge(r0, x, y), e1 = edge(r1, x, y), e2 = edge(r2, x, y);
-      if (t >= 0 && e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
-        const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
-        if (key < best_key || (key == best_key && t < best_id)) {
-          best_key = key;
-          best_id = t;
-        }
-      }
-    }
-  }
-  if (!inside) return;
 
   // argmin over all primitives returns index 0 of the scene when no candidate
   // has a finite (or NaN) depth (renderer/shader.py:207-217); the chosen
@@ -680,6 +626,91 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
   }
 }
 
+// One screen tile of TW x TH pixels with TW * TH / PPL threads: thread tid
+// owns the pixels tid + q * (TW * TH / PPL), q < PPL.  The tile's triangle
+// list is staged through LDS once and tested against all of a thread's pixels.
+template <int SHADER, int TW, int TH, int PPL>
+JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
+                        int32_t* lds_id) {
+  constexpr int NT = TW * TH / PPL;
+  const Frame& f = a.f;
+  const int32_t s = (int32_t)(gt / f.T);
+  const int32_t tile = (int32_t)(gt - (int64_t)s * f.T);
+  const int32_t tx = tile / f.tiles_y, ty = tile - tx * f.tiles_y;
+  const int tid = threadIdx.x;
+  const jr_scene& sc = a.scenes[s];
+  const float* vp = sc.viewport;
+  const float vp22 = vp[10], vp23 = vp[11];
+  int32_t pi[PPL], pj[PPL];
+  float px[PPL], py[PPL];
+  bool inside[PPL];
+  uint32_t best_key[PPL];
+  int32_t best_id[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int p = tid + q * NT;
+    pi[q] = tx * TW + p / TH;
+    pj[q] = ty * TH + p % TH;
+    inside[q] = (pi[q] < f.W) && (pj[q] < f.H);
+    px[q] = pix_x(pi[q], vp);
+    py[q] = pix_y(pj[q], vp);
+    // start from the small triangles already resolved by k_setup
+    best_key[q] = 0xFFFFFFFFu;
+    best_id[q] = 0x7FFFFFFF;
+    if (inside[q]) {
+      const uint64_t k0 = a.keys[(int64_t)s * f.W * f.H + (int64_t)pi[q] * f.H + pj[q]];
+      if (k0 != ~0ull) {
+        best_key[q] = (uint32_t)(k0 >> 32);
+        best_id[q] = (int32_t)(uint32_t)k0;
+      }
+    }
+  }
+
+  const int32_t cnt = a.counts[gt];
+  const int64_t start = a.offsets[gt];
+  const bool overflow = start + cnt > f.capacity;
+  // overflowing tile lists fall back to every triangle of the scene
+  const int32_t n_items = overflow ? (sc.tri_end - sc.tri_begin) : cnt;
+  for (int32_t base = 0; base < n_items; base += NT) {
+    const int32_t n = min(NT, n_items - base);
+    __syncthreads();
+    if (tid < n) {
+      const int32_t t = overflow ? (sc.tri_begin + base + tid) : a.bins[start + base + tid];
+      const float4 r0 = a.rec[4 * (int64_t)t], r1 = a.rec[4 * (int64_t)t + 1],
+                   r2 = a.rec[4 * (int64_t)t + 2];
+      const uint32_t fl = __float_as_uint(a.rec[4 * (int64_t)t + 3].y);
+      lds_rec[3 * tid + 0] = r0;
+      lds_rec[3 * tid + 1] = r1;
+      lds_rec[3 * tid + 2] = r2;
+      // triangles that are not kept or not front-facing never win the chooser
+      lds_id[tid] = ((fl & (kKeep | kFront)) == (kKeep | kFront)) ? t : -1;
+    }
+    __syncthreads();
+    for (int m = 0; m < n; ++m) {
+      const int32_t t = lds_id[m];
+      if (t < 0) continue;
+      const float4 r0 = lds_rec[3 * m + 0], r1 = lds_rec[3 * m + 1],
+                   r2 = lds_rec[3 * m + 2];
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        const float e0 = edge(r0, px[q], py[q]), e1 = edge(r1, px[q], py[q]),
+                    e2 = edge(r2, px[q], py[q]);
+        if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
+          const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
+          if (key < best_key[q] || (key == best_key[q] && t < best_id[q])) {
+            best_key[q] = key;
+            best_id[q] = t;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPL; ++q)
+    if (inside[q])
+      finish_pixel<SHADER>(a, sc, s, pi[q], pj[q], px[q], py[q], best_key[q], best_id[q]);
+}
+
 // Workgroups are dispatched to the 8 XCDs round-robin by group id, and each
 // XCD has its own L2.  With many scenes, group b takes a tile of a scene
 // s == b (mod 8), so every tile of a scene runs on one XCD and the scene's
@@ -696,24 +727,24 @@ JR_DEV int64_t xcd_item(const RasterArgs& a, int32_t x, int64_t k) {
   return (sl * kXcds + x) * a.f.T + (k - sl * a.f.T);
 }
 
-template <int SHADER, int TW, int TH>
-__global__ __launch_bounds__(TW* TH) void k_raster(RasterArgs a) {
-  __shared__ float4 lds_rec[3 * TW * TH];
-  __shared__ int32_t lds_id[TW * TH];
+template <int SHADER, int TW, int TH, int PPL>
+__global__ __launch_bounds__(TW* TH / PPL) void k_raster(RasterArgs a) {
+  __shared__ float4 lds_rec[3 * TW * TH / PPL];
+  __shared__ int32_t lds_id[TW * TH / PPL];
   int64_t gt = blockIdx.x;
   if (a.xcd_scenes) {
     gt = xcd_item(a, (int32_t)(blockIdx.x % kXcds), blockIdx.x / kXcds);
     if (gt < 0) return;
   }
-  raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
+  raster_tile<SHADER, TW, TH, PPL>(a, gt, lds_rec, lds_id);
 }
 
 // Several consecutive tiles per workgroup (a separate kernel: the loop raises
 // the register count of the single-tile form).
-template <int SHADER, int TW, int TH>
-__global__ __launch_bounds__(TW* TH) void k_raster_multi(RasterArgs a) {
-  __shared__ float4 lds_rec[3 * TW * TH];
-  __shared__ int32_t lds_id[TW * TH];
+template <int SHADER, int TW, int TH, int PPL>
+__global__ __launch_bounds__(TW* TH / PPL) void k_raster_multi(RasterArgs a) {
+  __shared__ float4 lds_rec[3 * TW * TH / PPL];
+  __shared__ int32_t lds_id[TW * TH / PPL];
   const int64_t b = blockIdx.x;
   const int64_t n_items = (int64_t)a.f.n_scenes * a.f.T;
   for (int32_t q = 0; q < a.tiles_per_group; ++q) {
@@ -725,17 +756,17 @@ __global__ __launch_bounds__(TW* TH) void k_raster_multi(RasterArgs a) {
       gt = b * a.tiles_per_group + q;
       if (gt >= n_items) break;
     }
-    raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
+    raster_tile<SHADER, TW, TH, PPL>(a, gt, lds_rec, lds_id);
   }
 }
 
 // Persistent form: a fixed grid drains the (scene, tile) queue.  Every group
 // exits once the shared counter passes the item count.
-template <int SHADER, int TW, int TH>
-__global__ __launch_bounds__(TW* TH) void k_raster_persistent(RasterArgs a,
-                                                              int64_t n_items) {
-  __shared__ float4 lds_rec[3 * TW * TH];
-  __shared__ int32_t lds_id[TW * TH];
+template <int SHADER, int TW, int TH, int PPL>
+__global__ __launch_bounds__(TW* TH / PPL) void k_raster_persistent(RasterArgs a,
+                                                                    int64_t n_items) {
+  __shared__ float4 lds_rec[3 * TW * TH / PPL];
+  __shared__ int32_t lds_id[TW * TH / PPL];
   __shared__ int32_t item;
   for (;;) {
     __syncthreads();
@@ -744,7 +775,7 @@ __global__ __launch_bounds__(TW* TH) void k_raster_persistent(RasterArgs a,
     const int32_t first = item;
     if (first >= n_items) break;
     const int32_t last = (int32_t)min((int64_t)first + a.chunk, n_items);
-    for (int32_t gt = first; gt < last; ++gt) raster_tile<SHADER, TW, TH>(a, gt, lds_rec, lds_id);
+    for (int32_t gt = first; gt < last; ++gt) raster_tile<SHADER, TW, TH, PPL>(a, gt, lds_rec, lds_id);
   }
 }
 
@@ -834,24 +865,29 @@ int32_t check_frame(const jr_frame* fr) {
     return fail(JR_E_INVALID, "bin_capacity out of range");
   if (fr->tiles_per_group < 0 || fr->tiles_per_group > 64)
     return fail(JR_E_INVALID, "tiles_per_group must be in [0, 64]");
+  const bool ok_ppl = fr->pixels_per_lane <= 1 ||
+                      (fr->pixels_per_lane == 4 && fr->tile_w * fr->tile_h == 256);
+  if (!ok_ppl)
+    return fail(JR_E_INVALID, "pixels_per_lane must be 0/1, or 4 with 16x16 or 8x32 tiles");
   return JR_OK;
 }
 
-template <int SHADER, int TW, int TH>
+template <int SHADER, int TW, int TH, int PPL>
 void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
                      hipStream_t st) {
+  constexpr int NT = TW * TH / PPL;
   if (persistent) {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int per_cu = 2048 / (TW * TH);  // up to 32 waves per CU
+    const int per_cu = 2048 / NT;  // up to 32 waves per CU
     const int64_t grid = std::min<int64_t>(n_items, (int64_t)std::max(cus, 1) * per_cu);
     // claim tiles in chunks: one queue atomic per chunk, about eight chunks
     // per group
     RasterArgs b = a;
     b.chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(32, n_items / (grid * 8)));
-    hipLaunchKernelGGL((k_raster_persistent<SHADER, TW, TH>), dim3((unsigned)grid),
-                       dim3(TW * TH), 0, st, b, n_items);
+    hipLaunchKernelGGL((k_raster_persistent<SHADER, TW, TH, PPL>), dim3((unsigned)grid),
+                       dim3(NT), 0, st, b, n_items);
   } else {
     RasterArgs b = a;
     const int64_t tpg = std::max(1, a.tiles_per_group);
@@ -863,22 +899,26 @@ void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
       grid = kXcds * ((per_xcd + tpg - 1) / tpg);
     }
     if (tpg == 1)
-      hipLaunchKernelGGL((k_raster<SHADER, TW, TH>), dim3((unsigned)grid),
-                         dim3(TW * TH), 0, st, b);
+      hipLaunchKernelGGL((k_raster<SHADER, TW, TH, PPL>), dim3((unsigned)grid), dim3(NT), 0,
+                         st, b);
     else
-      hipLaunchKernelGGL((k_raster_multi<SHADER, TW, TH>), dim3((unsigned)grid),
-                         dim3(TW * TH), 0, st, b);
+      hipLaunchKernelGGL((k_raster_multi<SHADER, TW, TH, PPL>), dim3((unsigned)grid),
+                         dim3(NT), 0, st, b);
   }
 }
 
 template <int SHADER>
 void launch_raster_s(const RasterArgs& a, int64_t n_items, bool persistent,
                      hipStream_t st) {
-  const int tw = a.f.TW, th = a.f.TH;
-  if (tw == 8 && th == 8) launch_raster_t<SHADER, 8, 8>(a, n_items, persistent, st);
-  else if (tw == 16 && th == 16) launch_raster_t<SHADER, 16, 16>(a, n_items, persistent, st);
-  else if (tw == 8 && th == 32) launch_raster_t<SHADER, 8, 32>(a, n_items, persistent, st);
-  else launch_raster_t<SHADER, 4, 16>(a, n_items, persistent, st);
+  const int tw = a.f.TW, th = a.f.TH, ppl = a.pixels_per_lane;
+  if (tw == 8 && th == 8) launch_raster_t<SHADER, 8, 8, 1>(a, n_items, persistent, st);
+  else if (tw == 16 && th == 16 && ppl == 4)
+    launch_raster_t<SHADER, 16, 16, 4>(a, n_items, persistent, st);
+  else if (tw == 16 && th == 16) launch_raster_t<SHADER, 16, 16, 1>(a, n_items, persistent, st);
+  else if (tw == 8 && th == 32 && ppl == 4)
+    launch_raster_t<SHADER, 8, 32, 4>(a, n_items, persistent, st);
+  else if (tw == 8 && th == 32) launch_raster_t<SHADER, 8, 32, 1>(a, n_items, persistent, st);
+  else launch_raster_t<SHADER, 4, 16, 1>(a, n_items, persistent, st);
 }
 
 void launch_raster(int shader, const RasterArgs& a, int64_t n_items,
@@ -1038,6 +1078,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.chunk = 1;
   a.xcd_scenes = 0;
   a.tiles_per_group = fr->tiles_per_group;
+  a.pixels_per_lane = fr->pixels_per_lane > 1 ? fr->pixels_per_lane : 1;
   a.init_buffers = fr->init_buffers;
   a.apply_offset = fr->apply_offset;
   a.zdef = fr->zbuffer_default;

@@ -24,9 +24,10 @@ def test_cache_roundtrip(tmp_path):
 
 def test_candidates_are_supported_tiles():
     assert (8, 8) in autotune.TILES
-    for (tw, th), form in autotune.DEFAULT_CANDIDATES:
-        assert (tw * th) % 64 == 0  # whole wavefronts per workgroup
+    for (tw, th), form, ppl in autotune.DEFAULT_CANDIDATES:
+        assert (tw * th // ppl) % 64 == 0  # whole wavefronts per workgroup
         assert form in autotune.FORMS and 0 <= form <= 64
+        assert ppl == 1 or tw * th == 256
 
 
 def test_cache_path_env(monkeypatch, tmp_path):
@@ -50,7 +51,7 @@ def test_autotune_small_batch(tmp_path):
     for tag, c in got.items():
         assert not c["cached"]
         assert len(c["timings"]) == len(autotune.DEFAULT_CANDIDATES)
-        assert (tuple(c["tile"]), c["form"]) in autotune.DEFAULT_CANDIDATES
+        assert (tuple(c["tile"]), c["form"], c["ppl"]) in autotune.DEFAULT_CANDIDATES
         assert br.passes[tag].persistent == (c["form"] == 0)
         assert br.passes[tag].tile == tuple(c["tile"])
     out = br.replay()

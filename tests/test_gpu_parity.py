@@ -51,6 +51,18 @@ def test_tiles_and_persistent(tile, persistent):
            f"tile{tile}/p{persistent}", tile=tile, persistent=persistent)
 
 
+@pytest.mark.parametrize("tile", [(16, 16), (8, 32)], ids=str)
+@pytest.mark.parametrize("form", [0, 1, 3])
+@pytest.mark.parametrize("shader", [DepthShader, PhongReflectionShadowTextureShader],
+                         ids=lambda s: s.__name__)
+def test_four_pixels_per_lane(tile, form, shader):
+    """One wavefront per 256-pixel tile, each lane owning 4 pixels, in the
+    persistent (0) and 1 / 3 tiles-per-group forms, on a frame whose size is
+    not a multiple of the tile."""
+    _check(lambda: scenes.soup_call(shader, 8, 70, 50, 120), f"ppl4/{tile}/{form}", tile=tile,
+           persistent=form == 0, tiles_per_group=max(form, 1), pixels_per_lane=4)
+
+
 @pytest.mark.parametrize("capacity", [0, 1, 17])
 def test_bin_overflow_falls_back(capacity):
     _check(lambda: scenes.soup_call(DepthShader, 3, 50, 40, 80), f"cap{capacity}",
