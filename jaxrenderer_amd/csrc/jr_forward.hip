@@ -44,48 +44,52 @@ namespace jr {
 // (u = 2^-24), so every accepted pixel's exact sample lies in the region
 // R = {a_k x + b_k y + c_k >= -delta_k} with delta_k = 8u(X|a| + Y|b| + |c|).
 // The range is the bounding box of R within the viewport's sample rectangle,
-// mapped to pixel indices with a 1/64-pixel allowance for the double-precision
-// rounding of the box itself.
+// mapped to pixel indices with a 1/64-pixel allowance.
 //
-// Common case: the three shifted lines meet pairwise in three points that
-// each satisfy the third inequality with room to spare; R is then exactly the
-// triangle they span, whose bounding box needs three 2x2 solves.  Otherwise
-// (R unbounded, empty or ill-conditioned) the viewport rectangle is clipped
-// by the three half-planes in double precision.
-JR_DEV bool shifted_triangle_box(const double a[3], const double b[3],
-                                 const double c[3], double* xmin, double* xmax,
-                                 double* ymin, double* ymax) {
-  // g bounds the relative rounding of a few double operations (2^-53 each)
-  constexpr double g = 8.0 / 9007199254740992.0;
-  double lo_x = INFINITY, hi_x = -INFINITY, lo_y = INFINITY, hi_y = -INFINITY;
+// When the three shifted lines meet pairwise in three points that each satisfy
+// the third inequality with room to spare, R is exactly the triangle they
+// span, whose box comes from three 2x2 solves in float, widened by an explicit
+// bound on their rounding (u = 2^-24 per operation; only systems far from
+// singular, |det| > 1e-3 of the product of the rows' 1-norms, and no
+// coefficient of magnitude in (0, 1e-12), so no intermediate is subnormal).
+// Otherwise (R unbounded, empty or ill-conditioned: a few triangles in a
+// thousand) the range is the whole viewport, and the exact per-tile test of
+// the binning keeps only the tiles that may hold a covered pixel.
+JR_DEV bool shifted_triangle_box_f(const float a[3], const float b[3], const float c[3],
+                                   float* xmin, float* xmax, float* ymin, float* ymax) {
+  constexpr float g = 8.0f / 16777216.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float m[3] = {fabsf(a[k]), fabsf(b[k]), fabsf(c[k])};
+    for (int q = 0; q < 3; ++q)
+      if (m[q] != 0.0f && !(m[q] >= 1e-12f)) return false;
+  }
+  float lo_x = INFINITY, hi_x = -INFINITY, lo_y = INFINITY, hi_y = -INFINITY;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int p = k, q = (k + 1) % 3, r = (k + 2) % 3;
-    const double det = a[p] * b[q] - a[q] * b[p];
-    const double scale = (fabs(a[p]) + fabs(b[p])) * (fabs(a[q]) + fabs(b[q]));
-    if (!(fabs(det) > 1e-6 * scale)) return false;
-    const double inv = 1.0 / det;
-    const double nx = b[p] * c[q] - b[q] * c[p];
-    const double ny = a[q] * c[p] - a[p] * c[q];
-    const double x = nx * inv, y = ny * inv;
-    // absolute error bounds of x and y from the rounding of det, the
-    // numerators and the division
-    const double ex = g * ((fabs(b[p] * c[q]) + fabs(b[q] * c[p])) * fabs(inv) +
-                           fabs(x) * (fabs(a[p] * b[q]) + fabs(a[q] * b[p])) * fabs(inv) +
-                           fabs(x));
-    const double ey = g * ((fabs(a[q] * c[p]) + fabs(a[p] * c[q])) * fabs(inv) +
-                           fabs(y) * (fabs(a[p] * b[q]) + fabs(a[q] * b[p])) * fabs(inv) +
-                           fabs(y));
-    const double v = a[r] * x + b[r] * y + c[r];
-    const double mag = fabs(a[r] * x) + fabs(b[r] * y) + fabs(c[r]);
-    // the true intersection must satisfy the third inequality
-    if (!(v > fabs(a[r]) * ex + fabs(b[r]) * ey + g * mag)) return false;
-    lo_x = fmin(lo_x, x - ex);
-    hi_x = fmax(hi_x, x + ex);
-    lo_y = fmin(lo_y, y - ey);
-    hi_y = fmax(hi_y, y + ey);
+    const float det = a[p] * b[q] - a[q] * b[p];
+    const float scale = (fabsf(a[p]) + fabsf(b[p])) * (fabsf(a[q]) + fabsf(b[q]));
+    if (!(fabsf(det) > 1e-3f * scale)) return false;
+    const float inv = 1.0f / det;
+    const float nx = b[p] * c[q] - b[q] * c[p];
+    const float ny = a[q] * c[p] - a[p] * c[q];
+    const float x = nx * inv, y = ny * inv;
+    const float cond = (fabsf(a[p] * b[q]) + fabsf(a[q] * b[p])) * fabsf(inv);
+    const float ex = g * ((fabsf(b[p] * c[q]) + fabsf(b[q] * c[p])) * fabsf(inv) +
+                          fabsf(x) * cond + fabsf(x));
+    const float ey = g * ((fabsf(a[q] * c[p]) + fabsf(a[p] * c[q])) * fabsf(inv) +
+                          fabsf(y) * cond + fabsf(y));
+    const float v = a[r] * x + b[r] * y + c[r];
+    const float mag = fabsf(a[r] * x) + fabsf(b[r] * y) + fabsf(c[r]);
+    if (!(v > fabsf(a[r]) * ex + fabsf(b[r]) * ey + g * mag)) return false;
+    lo_x = fminf(lo_x, x - ex);
+    hi_x = fmaxf(hi_x, x + ex);
+    lo_y = fminf(lo_y, y - ey);
+    hi_y = fmaxf(hi_y, y + ey);
   }
-  if (!(lo_x <= hi_x) || !(lo_y <= hi_y)) return false;
+  if (!(lo_x <= hi_x) || !(lo_y <= hi_y) || !(hi_x - lo_x < 1e30f) || !(hi_y - lo_y < 1e30f))
+    return false;
   *xmin = lo_x;
   *xmax = hi_x;
   *ymin = lo_y;
@@ -93,72 +97,11 @@ JR_DEV bool shifted_triangle_box(const double a[3], const double b[3],
   return true;
 }
 
-JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
-                        bool sane, const float* vp, const Frame& f) {
-  Rect full = {0, f.W - 1, 0, f.H - 1};
-  if (!sane || !(vp[0] > 0.0f) || !(vp[5] > 0.0f)) return full;
-  const double X0 = pix_x(0, vp), X1 = pix_x(f.W - 1, vp);
-  const double Y0 = pix_y(0, vp), Y1 = pix_y(f.H - 1, vp);
-  if (!(X0 <= X1) || !(Y0 <= Y1)) return full;
-  const double XM = fmax(fabs(X0), fabs(X1)), YM = fmax(fabs(Y0), fabs(Y1));
-  const float4 rs[3] = {r0, r1, r2};
-  double A[3], B[3], C[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    A[k] = rs[k].x;
-    B[k] = rs[k].y;
-    C[k] = (double)rs[k].z +
-           (8.0 / 16777216.0) * (XM * fabs(A[k]) + YM * fabs(B[k]) + fabs((double)rs[k].z)) +
-           1e-300;
-  }
+// Pixels whose exact NDC sample lies in [xmin, xmax] x [ymin, ymax], with a
+// 1/64-pixel allowance for the rounding of the box corners.
+JR_DEV Rect pixel_box(double xmin, double xmax, double ymin, double ymax, const float* vp,
+                      const Frame& f) {
   Rect r = {1, 0, 1, 0};
-  double xmin, xmax, ymin, ymax;
-  if (shifted_triangle_box(A, B, C, &xmin, &xmax, &ymin, &ymax)) {
-    xmin = fmax(xmin, X0);
-    xmax = fmin(xmax, X1);
-    ymin = fmax(ymin, Y0);
-    ymax = fmin(ymax, Y1);
-    if (!(xmin <= xmax) || !(ymin <= ymax)) return r;
-  } else {
-    double px[8], py[8], qx[8], qy[8];
-    px[0] = X0; py[0] = Y0;
-    px[1] = X1; py[1] = Y0;
-    px[2] = X1; py[2] = Y1;
-    px[3] = X0; py[3] = Y1;
-    int n = 4;
-    for (int k = 0; k < 3 && n > 0; ++k) {
-      int m = 0;
-      for (int v = 0; v < n; ++v) {
-        const int w = (v + 1 == n) ? 0 : v + 1;
-        const double dv = A[k] * px[v] + B[k] * py[v] + C[k];
-        const double dw = A[k] * px[w] + B[k] * py[w] + C[k];
-        if (dv >= 0.0) {
-          qx[m] = px[v];
-          qy[m] = py[v];
-          ++m;
-        }
-        if ((dv >= 0.0) != (dw >= 0.0)) {
-          const double t = dv / (dv - dw);
-          qx[m] = px[v] + t * (px[w] - px[v]);
-          qy[m] = py[v] + t * (py[w] - py[v]);
-          ++m;
-        }
-      }
-      n = m;
-      for (int v = 0; v < n; ++v) {
-        px[v] = qx[v];
-        py[v] = qy[v];
-      }
-    }
-    if (n == 0) return r;
-    xmin = px[0], xmax = px[0], ymin = py[0], ymax = py[0];
-    for (int v = 1; v < n; ++v) {
-      xmin = fmin(xmin, px[v]);
-      xmax = fmax(xmax, px[v]);
-      ymin = fmin(ymin, py[v]);
-      ymax = fmax(ymax, py[v]);
-    }
-  }
   const double vx = vp[0], vx3 = vp[3], vy = vp[5], vy3 = vp[7];
   constexpr double kSlack = 1.0 / 64.0;
   double i0 = ceil(xmin * vx + vx3 - kSlack), i1 = floor(xmax * vx + vx3 + kSlack);
@@ -173,6 +116,43 @@ JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
   r.ty0 = (int32_t)j0;
   r.ty1 = (int32_t)j1;
   return r;
+}
+
+JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
+                        bool sane, const float* vp, const Frame& f) {
+  Rect full = {0, f.W - 1, 0, f.H - 1};
+  if (!sane || !(vp[0] > 0.0f) || !(vp[5] > 0.0f)) return full;
+  const double X0 = pix_x(0, vp), X1 = pix_x(f.W - 1, vp);
+  const double Y0 = pix_y(0, vp), Y1 = pix_y(f.H - 1, vp);
+  if (!(X0 <= X1) || !(Y0 <= Y1)) return full;
+  const double XM = fmax(fabs(X0), fabs(X1)), YM = fmax(fabs(Y0), fabs(Y1));
+  const float4 rs[3] = {r0, r1, r2};
+  Rect r = {1, 0, 1, 0};
+  double xmin, xmax, ymin, ymax;
+  // float attempt: the shift is rounded up by a further u |C| (the bound above
+  // has 2.9u to spare), so the float lines enclose the same region R
+  float Af[3], Bf[3], Cf[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Af[k] = rs[k].x;
+    Bf[k] = rs[k].y;
+    const float d = (8.0f / 16777216.0f) *
+                    ((float)XM * fabsf(rs[k].x) + (float)YM * fabsf(rs[k].y) + fabsf(rs[k].z));
+    Cf[k] = rs[k].z + d * 1.0001f + 1e-37f;
+  }
+  float fxmin, fxmax, fymin, fymax;
+  if (shifted_triangle_box_f(Af, Bf, Cf, &fxmin, &fxmax, &fymin, &fymax)) {
+    xmin = fmax((double)fxmin, X0);
+    xmax = fmin((double)fxmax, X1);
+    ymin = fmax((double)fymin, Y0);
+    ymax = fmin((double)fymax, Y1);
+    if (!(xmin <= xmax) || !(ymin <= ymax)) return r;
+    return pixel_box(xmin, xmax, ymin, ymax, vp, f);
+  }
+  // R unbounded, empty or ill-conditioned (a few triangles in a thousand):
+  // the whole viewport; the exact per-tile test then keeps only the tiles
+  // that may hold a covered pixel
+  return full;
 }
 
 JR_DEV Rect tiles_of(const Rect& p, const Frame& f) {
@@ -209,18 +189,19 @@ JR_DEV int32_t zero_pixel(float v3, int32_t lo, int32_t hi) {
 }
 
 JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
-                           bool sane, const float* vp, const Frame& f,
-                           int32_t tx, int32_t ty) {
+                           bool sane, const float* vp, const float* __restrict__ sx,
+                           const Frame& f, int32_t tx, int32_t ty) {
+  // sx: the scene's sample table (pix_x for i < W, then pix_y)
   const int32_t i0 = tx * f.TW, i1 = min(i0 + f.TW, f.W) - 1;
   const int32_t j0 = ty * f.TH, j1 = min(j0 + f.TH, f.H) - 1;
-  float xs[3] = {pix_x(i0, vp), pix_x(i1, vp), 0.0f};
-  float ys[3] = {pix_y(j0, vp), pix_y(j1, vp), 0.0f};
+  float xs[3] = {sx[i0], sx[i1], 0.0f};
+  float ys[3] = {sx[f.W + j0], sx[f.W + j1], 0.0f};
   int nx = 2, ny = 2;
   if (!sane) {
     if (!(vp[0] > 0.0f) || !(vp[5] > 0.0f)) return true;
     const int32_t iz = zero_pixel(vp[3], i0, i1), jz = zero_pixel(vp[7], j0, j1);
-    if (iz >= 0) xs[nx++] = pix_x(iz, vp);
-    if (jz >= 0) ys[ny++] = pix_y(jz, vp);
+    if (iz >= 0) xs[nx++] = sx[iz];
+    if (jz >= 0) ys[ny++] = sx[f.W + jz];
   }
   const float4 rs[3] = {r0, r1, r2};
 #pragma unroll
@@ -296,19 +277,19 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     TriSetup ts;
     tri_setup(in, sc, t, ts);
     const bool keep = ts.keep, front = ts.front, sane = ts.sane;
-    const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
     const float4 r0 = rec_col(ts, 0);
     const float4 r1 = rec_col(ts, 1);
     const float4 r2 = rec_col(ts, 2);
     Rect r = {1, 0, 1, 0};
-    rec[4 * (int64_t)t + 0] = r0;
-    rec[4 * (int64_t)t + 1] = r1;
-    rec[4 * (int64_t)t + 2] = r2;
-    rec[4 * (int64_t)t + 3] = make_float4(ts.det, __uint_as_float(flags), 0.0f, 0.0f);
     Rect px = r;
     // a NaN coefficient makes that edge value NaN at every pixel: never inside
     const bool nan_col = has_nan_column(r0) || has_nan_column(r1) || has_nan_column(r2);
     if (keep && front && !nan_col) px = pixel_range(r0, r1, r2, sane, sc.viewport, f);
+    const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
+    rec[4 * (int64_t)t + 0] = r0;
+    rec[4 * (int64_t)t + 1] = r1;
+    rec[4 * (int64_t)t + 2] = r2;
+    rec[4 * (int64_t)t + 3] = make_float4(ts.det, __uint_as_float(flags), 0.0f, 0.0f);
     const int64_t area = (px.tx0 > px.tx1)
         ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
     if (area > kSmallPixels) r = tiles_of(px, f);
@@ -327,9 +308,10 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
       wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
     } else {
       int32_t* cnt = counts + (int64_t)s * f.T;
+      const float* sxs = samples + (int64_t)s * (f.W + f.H);
       for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
         for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
-          if (tile_may_cover(r0, r1, r2, sane, sc.viewport, f, tx, ty))
+          if (tile_may_cover(r0, r1, r2, sane, sc.viewport, sxs, f, tx, ty))
             atomicAdd(cnt + tx * f.tiles_y + ty, 1);
     }
   }
@@ -383,7 +365,7 @@ __global__ __launch_bounds__(256) void k_wide(
     const Rect* __restrict__ rects, const int32_t* __restrict__ wide,
     const int32_t* __restrict__ wide_count, int32_t* __restrict__ counts,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ bins) {
+    int32_t* __restrict__ bins, const float* __restrict__ samples) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -401,7 +383,8 @@ __global__ __launch_bounds__(256) void k_wide(
     const int64_t tbase = (int64_t)s * f.T;
     for (int32_t q = lane; q < n_t; q += 64) {
       const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
-      if (!tile_may_cover(r0, r1, r2, sane, vp, f, tx, ty)) continue;
+      if (!tile_may_cover(r0, r1, r2, sane, vp, samples + (int64_t)s * (f.W + f.H), f, tx, ty))
+        continue;
       const int64_t gt = tbase + tx * f.tiles_y + ty;
       if (BIN) {
         const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
@@ -487,21 +470,23 @@ __global__ __launch_bounds__(256) void k_bin(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     const float4* __restrict__ rec, const Rect* __restrict__ rects,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ bins) {
+    int32_t* __restrict__ bins, const float* __restrict__ samples) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= in.n_tris) return;
   const Rect r = rects[t];
   if (r.tx0 > r.tx1) return;
   if ((int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1) > kWideTiles) return;  // k_wide
+  const uint32_t fl = __float_as_uint(rec[4 * (int64_t)t + 3].y);
   const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
   const float* vp = scenes[s].viewport;
   const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
                r2 = rec[4 * (int64_t)t + 2];
-  const bool sane = (__float_as_uint(rec[4 * (int64_t)t + 3].y) & kSane) != 0;
+  const bool sane = (fl & kSane) != 0;
   const int64_t tbase = (int64_t)s * f.T;
+  const float* sxs = samples + (int64_t)s * (f.W + f.H);
   for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
     for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
-      if (tile_may_cover(r0, r1, r2, sane, vp, f, tx, ty)) {
+      if (tile_may_cover(r0, r1, r2, sane, vp, sxs, f, tx, ty)) {
         const int64_t gt = tbase + tx * f.tiles_y + ty;
         const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
         if (pos < f.capacity) bins[pos] = t;
@@ -1048,7 +1033,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   timer.mark(2);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
-                       rects, wide, wide_count, counts, offsets, cursor, bins);
+                       rects, wide, wide_count, counts, offsets, cursor, bins, samples);
   timer.mark(3);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
                      dim3(kScanThreads), 0, st, counts, offsets, block_sums, L.n_tiles);
@@ -1059,9 +1044,9 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   timer.mark(4);
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rec, rects, offsets, cursor, bins);
+                       scenes, f, rec, rects, offsets, cursor, bins, samples);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
-                       rects, wide, wide_count, counts, offsets, cursor, bins);
+                       rects, wide, wide_count, counts, offsets, cursor, bins, samples);
   }
   timer.mark(5);
   RasterArgs a;
