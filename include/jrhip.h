@@ -240,7 +240,11 @@ int32_t jr_stage_times(float* ms, int32_t max_calls);
  * (renderer/pipeline.py:105).  Supported shaders: DEPTH, GOURAUD,
  * GOURAUD_TEXTURE, PHONG, PHONG_REFLECTION, PHONG_REFLECTION_SHADOW.
  * frame->width/height/n_scenes/shader/tile_w/tile_h must match the forward
- * call; the workspace is sized by jr_backward_workspace_size. */
+ * call; the workspace is sized by jr_backward_workspace_size.  The workspace
+ * carries state from call to call (its per-triangle accumulators are left
+ * zero, so a call clears only what it used): keep one per stream, do not
+ * write to it between calls; a new, resized or foreign workspace is detected
+ * by its header and cleared in full. */
 int64_t jr_backward_workspace_size(const jr_frame* frame, int32_t n_tris);
 /* Message of the calling thread's last failed jr_backward. */
 const char* jr_backward_last_error(void);

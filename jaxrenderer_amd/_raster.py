@@ -143,8 +143,11 @@ _ws_lock = threading.Lock()
 _workspaces: Dict[Tuple[int, int], _Workspace] = {}
 
 
-def workspace_for(stream: torch.cuda.Stream, nbytes: int, dev: torch.device) -> torch.Tensor:
-    key = (dev.index, stream.cuda_stream)
+def workspace_for(stream: torch.cuda.Stream, nbytes: int, dev: torch.device,
+                  kind: str = "forward") -> torch.Tensor:
+    """A cached workspace per (device, stream, kind).  The backward pass keeps
+    its own: it carries zeroed accumulators from one call to the next."""
+    key = (dev.index, stream.cuda_stream, kind)
     with _ws_lock:
         ws = _workspaces.setdefault(key, _Workspace())
         if ws.buf is None or ws.buf.numel() < nbytes:
@@ -317,7 +320,7 @@ def backward(
         setattr(jg, name, _ptr(grads.get(name), name, torch.float32, dev))
     need = backward_workspace_size(width=width, height=height, n_scenes=n_scenes,
                                    shader=shader, n_tris=n_tris)
-    ws = workspace if workspace is not None else workspace_for(stream, need, dev)
+    ws = workspace if workspace is not None else workspace_for(stream, need, dev, "backward")
     if ws.numel() < need:
         raise ValueError(f"backward workspace holds {ws.numel()} bytes, {need} needed")
     rc = L.jr_backward(

@@ -16,13 +16,16 @@
 // and fragment attributes with the forward's own code (jr_common.h,
 // jr_shade.h), so they are bit-identical to what the forward used.
 //
-//   k_bwd_clear   zero the per-triangle accumulators
+//   k_bwd_prepare / k_bwd_fill  check the workspace header; clear the
+//                 per-triangle flags and accumulators only for a workspace that
+//                 no earlier call left zero (each call clears what it used)
 //   k_bwd_pixels  one 64-lane group per 8x8 tile: per pixel, shading backward
 //                 (texture / specular scatter, per-scene reductions) and the
 //                 visibility backward (dL/dM^-1, dL/dz_clip, per-corner varying
 //                 cotangents), grouped per triangle in LDS before one set of
 //                 global atomics per (tile, triangle)
-//   k_bwd_tris    one thread per triangle: M^-1 -> M -> clip -> positions and
+//   k_bwd_tris    one thread per triangle, work for the touched ones: M^-1 -> M ->
+//                 clip -> positions and
 //                 the vertex stage of each corner (normals, colours, light,
 //                 camera matrices)
 #include <hip/hip_runtime.h>
@@ -385,9 +388,17 @@ struct BwdArgs {
   const float* d_canvas;
   const float* d_zbuffer;
   const int32_t* prim_id;
-  float* tacc;
+  float* tacc;              // kAcc floats per triangle, zero between calls
+  int32_t* touched;         // per triangle: 1 once a pixel adds to it, zero between calls
+  uint32_t* header;         // workspace header (kHdr* words)
   int32_t W, H, tiles_x, tiles_y, T, n_scenes;
 };
+
+// Workspace header words.  The accumulators and flags are left zero by every
+// call (k_bwd_tris clears what it consumed), so a call only clears them in
+// full when the header does not describe this workspace yet.
+constexpr int kHdrMagicLo = 0, kHdrMagicHi = 1, kHdrTris = 2, kHdrFill = 3;
+constexpr uint32_t kMagicLo = 0x6a724277u, kMagicHi = 0x73706b31u;
 
 template <int SHADER>
 __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
@@ -491,11 +502,13 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
     for (int q = 0; q < kAcc; ++q)
       if (vals[q] != 0.0f) atomicAdd(&acc[leader][q], vals[q]);
   __syncthreads();
-  if (t >= 0 && leader == lane)
+  if (t >= 0 && leader == lane) {
+    a.touched[t] = 1;  // k_bwd_tris only visits touched triangles
     for (int q = 0; q < kAcc; ++q) {
       const float v = acc[lane][q];
       if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + q, v);
     }
+  }
   // per-scene parameters: one reduction per tile
   if (a.gr.d_scene) {
     for (int q = 0; q < S_COUNT; ++q) {
@@ -572,22 +585,27 @@ JR_DEV void vertex_bwd(const BwdArgs& a, const jr_scene& sc, int32_t slot, const
     for (int q = 0; q < 3; ++q) atomicAdd(a.gr.d_normals + 3 * (int64_t)nrow + q, dn[q]);
 }
 
+// Triangle t (-1: none) of the calling lane; the accumulators are read and
+// cleared only when `touched`.  Every lane computes its scene, so that a
+// wavefront of one scene reduces its camera and light gradients once.
 template <int SHADER>
-__global__ __launch_bounds__(256) void k_bwd_tris(BwdArgs a) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, bool touched) {
   float dW[16], dE[16], dLd[3], dLc[3];
   for (int q = 0; q < 16; ++q) dW[q] = dE[q] = 0.0f;
   for (int q = 0; q < 3; ++q) dLd[q] = dLc[q] = 0.0f;
+  if (__ballot(touched) == 0) return;  // nothing to add for this wavefront
   int32_t s = -1;
-  if (t < a.in.n_tris) {
-    const float* A = a.tacc + (int64_t)t * kAcc;
+  if (t >= 0) s = scene_of_tri_wave(a.scenes, a.n_scenes, t);
+  if (touched) {
+    float* A = a.tacc + (int64_t)t * kAcc;
     bool any = false;
     float v[kAcc];
     for (int q = 0; q < kAcc; ++q) {
       v[q] = A[q];
+      A[q] = 0.0f;
       any = any || (v[q] != 0.0f);
     }
-    s = scene_of_tri(a.scenes, a.n_scenes, t);
+    a.touched[t] = 0;
     if (any) {
       const jr_scene& sc = a.scenes[s];
       TriSetup ts;
@@ -665,10 +683,35 @@ __global__ __launch_bounds__(256) void k_bwd_tris(BwdArgs a) {
   }
 }
 
-__global__ void k_bwd_clear(float* __restrict__ p, int64_t n) {
+// One thread per triangle; only the touched ones read their accumulators,
+// which are then cleared with the flag for the next call.
+template <int SHADER>
+__global__ __launch_bounds__(256) void k_bwd_tris(BwdArgs a) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < a.in.n_tris;
+  bwd_tri<SHADER>(a, valid ? t : -1, valid && a.touched[t] != 0);
+}
+
+// Reads the workspace header: if it does not describe this workspace (new,
+// resized, or used for another triangle count), asks k_bwd_fill to clear the
+// flags and accumulators.
+__global__ void k_bwd_prepare(uint32_t* __restrict__ hdr, int32_t n_tris) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const bool ok = hdr[kHdrMagicLo] == kMagicLo && hdr[kHdrMagicHi] == kMagicHi &&
+                  hdr[kHdrTris] == (uint32_t)n_tris;
+  hdr[kHdrFill] = ok ? 0u : 1u;
+  hdr[kHdrMagicLo] = kMagicLo;
+  hdr[kHdrMagicHi] = kMagicHi;
+  hdr[kHdrTris] = (uint32_t)n_tris;
+}
+
+// Zeroes n 32-bit words after the header when k_bwd_prepare asked for it.
+__global__ void k_bwd_fill(const uint32_t* __restrict__ hdr, uint32_t* __restrict__ p,
+                           int64_t n) {
+  if (hdr[kHdrFill] == 0u) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0.0f;
+    p[i] = 0u;
 }
 
 thread_local std::string g_bwd_err;
@@ -685,6 +728,21 @@ void launch_bwd(const BwdArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_bwd_tris<SHADER>, dim3((a.in.n_tris + 255) / 256), dim3(256), 0, st, a);
 }
 
+// Workspace: header (256 B), touched flags (4 B per triangle), accumulators
+// (4 kAcc B per triangle).
+struct BwdLayout {
+  int64_t touched, tacc, total;
+};
+
+BwdLayout bwd_layout(int32_t n_tris) {
+  const int64_t n = std::max(n_tris, 1);
+  BwdLayout L;
+  L.touched = 256;
+  L.tacc = L.touched + (4 * n + 255) / 256 * 256;
+  L.total = L.tacc + (4 * (int64_t)kAcc * n + 255) / 256 * 256;
+  return L;
+}
+
 }  // namespace jr
 
 using namespace jr;
@@ -693,7 +751,7 @@ extern "C" {
 
 int64_t jr_backward_workspace_size(const jr_frame* frame, int32_t n_tris) {
   if (!frame || n_tris < 0) return -1;
-  return ((4 * (int64_t)kAcc * std::max(n_tris, 1)) + 255) / 256 * 256;
+  return bwd_layout(n_tris).total;
 }
 
 const char* jr_backward_last_error(void) { return g_bwd_err.c_str(); }
@@ -735,7 +793,11 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
   a.d_canvas = (sh == JR_SHADER_DEPTH) ? nullptr : d_canvas;
   a.d_zbuffer = d_zbuffer;
   a.prim_id = prim_id;
-  a.tacc = static_cast<float*>(workspace);
+  const BwdLayout WL = bwd_layout(in->n_tris);
+  char* ws = static_cast<char*>(workspace);
+  a.header = reinterpret_cast<uint32_t*>(ws);
+  a.touched = reinterpret_cast<int32_t*>(ws + WL.touched);
+  a.tacc = reinterpret_cast<float*>(ws + WL.tacc);
   a.W = fr->width;
   a.H = fr->height;
   a.tiles_x = (a.W + 7) / 8;
@@ -743,9 +805,12 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
   a.T = a.tiles_x * a.tiles_y;
   a.n_scenes = fr->n_scenes;
   if (in->n_tris == 0) return JR_OK;
-  const int64_t nacc = (int64_t)kAcc * in->n_tris;
-  hipLaunchKernelGGL(k_bwd_clear, dim3((unsigned)std::min<int64_t>((nacc + 255) / 256, 8192)),
-                     dim3(256), 0, st, a.tacc, nacc);
+  // flags and accumulators: everything after the header
+  const int64_t nwords = (WL.total - WL.touched) / 4;
+  hipLaunchKernelGGL(k_bwd_prepare, dim3(1), dim3(64), 0, st, a.header, in->n_tris);
+  hipLaunchKernelGGL(k_bwd_fill, dim3((unsigned)std::min<int64_t>((nwords + 255) / 256, 8192)),
+                     dim3(256), 0, st, a.header,
+                     reinterpret_cast<uint32_t*>(ws + WL.touched), nwords);
   switch (sh) {
     case JR_SHADER_DEPTH: launch_bwd<JR_SHADER_DEPTH>(a, st); break;
     case JR_SHADER_GOURAUD: launch_bwd<JR_SHADER_GOURAUD>(a, st); break;
