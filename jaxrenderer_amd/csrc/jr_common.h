@@ -54,6 +54,10 @@ struct Frame {
   int32_t W, H, TW, TH, tiles_x, tiles_y, T;
   int32_t n_scenes;
   int64_t capacity;
+  // depth-only pass whose output is the depth alone (no primitive ids or
+  // barycentrics) plus a nonzero offset: the small-triangle keys are 32-bit
+  // depth keys kept in the zbuffer itself
+  int32_t narrow;
 };
 
 JR_DEV int32_t scene_of_tri(const jr_scene* __restrict__ scenes, int32_t n,

@@ -222,16 +222,19 @@ JR_DEV bool has_nan_column(const float4& r) {
   return (r.x != r.x) || (r.y != r.y) || (r.z != r.z);
 }
 
-// Clears the tile counters (n32 int32), sets the per-pixel keys (n64
-// uint64) to "no candidate" and fills every scene's sample table: the NDC
+// Clears the tile counters (n32 int32), sets the per-pixel keys (n64 uint64,
+// or nk32 uint32 in the narrow form) to "no candidate" and fills every scene's
+// sample table: the NDC
 // coordinates pix_x(i) for i < W followed by pix_y(j) for j < H.
 __global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __restrict__ p64,
-                        int64_t n64, const jr_scene* __restrict__ scenes, Frame f,
+                        int64_t n64, uint32_t* __restrict__ k32, int64_t nk32,
+                        const jr_scene* __restrict__ scenes, Frame f,
                         float* __restrict__ samples) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   for (int64_t i = first; i < n32; i += stride) p32[i] = 0;
   for (int64_t i = first; i < n64; i += stride) p64[i] = ~0ull;
+  for (int64_t i = first; i < nk32; i += stride) k32[i] = ~0u;
   const int32_t per = f.W + f.H;
   for (int64_t i = first; i < (int64_t)f.n_scenes * per; i += stride) {
     const int32_t s = (int32_t)(i / per), q = (int32_t)(i - (int64_t)s * per);
@@ -250,6 +253,23 @@ __global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __rest
 // every lane (the owning triangle found by a binary search over the prefix
 // sums in LDS).  A wavefront therefore takes sum(areas) / 64 steps instead of
 // max(area) steps of one lane per triangle.
+// True when the window depth ((e0 z0 + e1 z1) + e2 z2) vp22 + vp23 stays
+// below 1e37 in magnitude for every sample of the pixel box: then no product
+// overflows and, with finite z_k, the depth is finite (never NaN) wherever
+// the pixel test passes.  |e_k| <= |a_k| X + |b_k| Y + |c_k| over the box.
+JR_DEV bool depth_bounded(const float4& r0, const float4& r1, const float4& r2,
+                          const Rect& px, const float* vp, const float* sx, const Frame& f) {
+  const float X = fmaxf(fabsf(sx[px.tx0]), fabsf(sx[px.tx1]));
+  const float Y = fmaxf(fabsf(sx[f.W + px.ty0]), fabsf(sx[f.W + px.ty1]));
+  const float4 rs[3] = {r0, r1, r2};
+  float b = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    b += (fabsf(rs[k].x) * X + fabsf(rs[k].y) * Y + fabsf(rs[k].z)) * 1.001f * fabsf(rs[k].w);
+  b = b * fabsf(vp[10]) + fabsf(vp[11]);
+  return b < 1e37f;
+}
+
 constexpr int kSetupThreads = 256;
 
 struct SmallStage {
@@ -264,7 +284,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     float4* __restrict__ rec, Rect* __restrict__ rects,
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
     int32_t* __restrict__ wide, int32_t* __restrict__ wide_count,
-    const float* __restrict__ samples) {
+    const float* __restrict__ samples, uint32_t* __restrict__ zkeys) {
   __shared__ SmallStage sm;
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < in.n_tris;
@@ -292,9 +312,14 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     rec[4 * (int64_t)t + 3] = make_float4(ts.det, __uint_as_float(flags), 0.0f, 0.0f);
     const int64_t area = (px.tx0 > px.tx1)
         ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
-    if (area > kSmallPixels) r = tiles_of(px, f);
+    // the narrow form reconstructs a small winner's depth from its key, so its
+    // small triangles must have a finite depth at every pixel of the box
+    const bool small = area > 0 && area <= kSmallPixels &&
+                       (!f.narrow || depth_bounded(r0, r1, r2, px, sc.viewport,
+                                                   samples + (int64_t)s * (f.W + f.H), f));
+    if (area > 0 && !small) r = tiles_of(px, f);
     rects[t] = r;
-    if (area > 0 && area <= kSmallPixels) {
+    if (small) {
       n_small = (int32_t)area;
       sm.r[0][threadIdx.x] = r0;
       sm.r[1][threadIdx.x] = r1;
@@ -351,8 +376,12 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
       const float* vp = scenes[box.w].viewport;
       const uint32_t key = depth_key(win_depth(q0, q1, q2, e0, e1, e2, vp[10], vp[11]));
-      atomicMin((unsigned long long*)(keys + ((int64_t)box.w * f.W + i) * f.H + j),
-                (unsigned long long)pack_key(key, sm.tri[o]));
+      const int64_t pix = ((int64_t)box.w * f.W + i) * f.H + j;
+      if (f.narrow)
+        atomicMin(zkeys + pix, key);
+      else
+        atomicMin((unsigned long long*)(keys + pix),
+                  (unsigned long long)pack_key(key, sm.tri[o]));
     }
   }
 }
@@ -521,6 +550,13 @@ JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int
   const Frame& f = a.f;
   const float* vp = sc.viewport;
   const float vp22 = vp[10], vp23 = vp[11];
+  if (best_id == -1) {
+    // narrow form, small winner: its depth is the key's (finite by
+    // construction); the depth shader keeps every fragment
+    const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
+    a.out.zbuffer[pix] = key_depth(best_key) + a.depth_offset;
+    return;
+  }
 
   // argmin over all primitives returns index 0 of the scene when no candidate
   // has a finite (or NaN) depth (renderer/shader.py:207-217); the chosen
@@ -639,14 +675,24 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
     inside[q] = (pi[q] < f.W) && (pj[q] < f.H);
     px[q] = pix_x(pi[q], vp);
     py[q] = pix_y(pj[q], vp);
-    // start from the small triangles already resolved by k_setup
+    // start from the small triangles already resolved by k_setup (in the
+    // narrow form their id is not kept: -1, which no equal key displaces)
     best_key[q] = 0xFFFFFFFFu;
     best_id[q] = 0x7FFFFFFF;
     if (inside[q]) {
-      const uint64_t k0 = a.keys[(int64_t)s * f.W * f.H + (int64_t)pi[q] * f.H + pj[q]];
-      if (k0 != ~0ull) {
-        best_key[q] = (uint32_t)(k0 >> 32);
-        best_id[q] = (int32_t)(uint32_t)k0;
+      const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)pi[q] * f.H + pj[q];
+      if (f.narrow) {
+        const uint32_t k0 = reinterpret_cast<const uint32_t*>(a.out.zbuffer)[pix];
+        if (k0 != ~0u) {
+          best_key[q] = k0;
+          best_id[q] = -1;
+        }
+      } else {
+        const uint64_t k0 = a.keys[pix];
+        if (k0 != ~0ull) {
+          best_key[q] = (uint32_t)(k0 >> 32);
+          best_id[q] = (int32_t)(uint32_t)k0;
+        }
       }
     }
   }
@@ -1000,6 +1046,13 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   f.T = f.tiles_x * f.tiles_y;
   f.n_scenes = fr->n_scenes;
   f.capacity = fr->bin_capacity;
+  {
+    uint32_t ob;
+    std::memcpy(&ob, &fr->depth_offset, 4);
+    // -0 would keep the sign of a -0 depth, which a key cannot tell apart
+    f.narrow = fr->shader == JR_SHADER_DEPTH && !out->prim_id && !out->bary &&
+               fr->apply_offset && fr->init_buffers && ob != 0x80000000u;
+  }
 
   float4* rec = reinterpret_cast<float4*>(ws + L.rec);
   Rect* rects = reinterpret_cast<Rect*>(ws + L.rects);
@@ -1022,14 +1075,16 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   const int64_t nz = (L.offsets - L.counts) / 4;
   const int64_t nclr = std::max(std::max(nz, L.n_pix),
                                 (int64_t)(f.W + f.H) * f.n_scenes);
+  uint32_t* zkeys = reinterpret_cast<uint32_t*>(out->zbuffer);
   hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<int64_t>((nclr + 255) / 256, 8192)),
-                     dim3(256), 0, st, counts, nz, keys, L.n_pix, scenes, f, samples);
+                     dim3(256), 0, st, counts, nz, keys, f.narrow ? (int64_t)0 : L.n_pix,
+                     zkeys, f.narrow ? L.n_pix : (int64_t)0, scenes, f, samples);
   timer.mark(1);
   const int n_tris = in->n_tris;
   if (n_tris > 0)
     hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
                        dim3(kSetupThreads), 0, st, *in,
-                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples);
+                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples, zkeys);
   timer.mark(2);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,

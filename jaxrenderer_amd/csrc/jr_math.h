@@ -211,4 +211,10 @@ JR_DEV uint32_t depth_key(float z) {
 
 constexpr uint32_t kInfKey = 0xFF800000u;  // depth_key(+inf)
 
+// The depth of a key from depth_key (not 0: NaN has no single inverse; -0
+// comes back as +0).
+JR_DEV float key_depth(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
 }  // namespace jr

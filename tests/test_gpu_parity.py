@@ -117,3 +117,31 @@ def test_many_scenes_xcd_mapping_and_tiles_per_group(tiles_per_group):
     scs, sm = workloads.config3(n_scenes=70, seed=12, width=40, height=30, shadow_size=24)
     _check(lambda: batch.plan_batch(scs, 40, 30, sm), f"xcd{tiles_per_group}", tile=(16, 16),
            tiles_per_group=tiles_per_group)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("tile,ppl", [((8, 8), 1), ((16, 16), 4)], ids=str)
+def test_depth_only_offset_pass(seed, tile, ppl):
+    """A depth-shader pass with an offset and no id outputs (a shadow map):
+    small triangles keep 32-bit depth keys in the output buffer itself.  The
+    depth must still match the oracle bit for bit, on tiny and pixel-aligned
+    triangles and on config-3 shadow maps."""
+    from jaxrenderer_amd import batch, workloads
+
+    def calls():
+        c = scenes.tiny_soup_call(seed)[0]
+        c.apply_offset, c.depth_offset, c.zbuffer_default = True, 0.05, 3.0e38
+        return [c]
+
+    for make, label in ((calls, f"tinyshadow{seed}"),):
+        ref = run_oracle_plan(make(), prim_id=False, bary=False)
+        got = _plan.run_plan(make(), tile=tile, pixels_per_lane=ppl)
+        torch.cuda.synchronize()
+        for tag in ref:
+            assert_parity(got[tag], ref[tag], label=f"{label}/{tag}")
+    scs, sm = workloads.config3(n_scenes=3, seed=20 + seed)
+    shadow = [c for c in batch.plan_batch(scs, 84, 84, sm) if c.tag == "shadow"]
+    ref = run_oracle_plan(shadow, prim_id=False, bary=False)
+    got = _plan.run_plan(shadow, tile=tile, pixels_per_lane=ppl)
+    torch.cuda.synchronize()
+    assert_parity(got["shadow"], ref["shadow"], label=f"c3shadow{seed}")
