@@ -533,6 +533,7 @@ struct RasterArgs {
   const int32_t* offsets;
   const int32_t* bins;
   const uint64_t* keys;
+  const float* samples;  // per scene: pix_x(i) for i < W, then pix_y(j)
   int32_t* work_counter;
   int32_t chunk;  // tiles claimed per work-queue step (persistent form)
   int32_t xcd_scenes;  // one-group-per-tile form: scenes kept on one XCD each
@@ -555,6 +556,17 @@ JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int
     // construction); the depth shader keeps every fragment
     const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
     a.out.zbuffer[pix] = key_depth(best_key) + a.depth_offset;
+    return;
+  }
+  if ((SHADER == JR_SHADER_DEPTH || SHADER == JR_SHADER_VISIBILITY) && !a.out.bary &&
+      best_key < kInfKey && best_key != 0u && best_key != 0x80000000u) {
+    // a depth-only winner whose key is neither NaN nor zero gives its depth
+    // back exactly; no barycentrics are needed, and these shaders keep every
+    // fragment
+    const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
+    const float z = key_depth(best_key);
+    a.out.zbuffer[pix] = a.apply_offset ? z + a.depth_offset : z;
+    if (a.out.prim_id) a.out.prim_id[pix] = best_id;
     return;
   }
 
@@ -667,14 +679,17 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
   bool inside[PPL];
   uint32_t best_key[PPL];
   int32_t best_id[PPL];
+  // with NT a multiple of TH all of a thread's pixels share one j (and y)
+  constexpr bool kOneRow = (NT % TH) == 0;
+  const float* sx = a.samples + (int64_t)s * (f.W + f.H);
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     const int p = tid + q * NT;
     pi[q] = tx * TW + p / TH;
-    pj[q] = ty * TH + p % TH;
+    pj[q] = ty * TH + (kOneRow ? tid % TH : p % TH);
     inside[q] = (pi[q] < f.W) && (pj[q] < f.H);
-    px[q] = pix_x(pi[q], vp);
-    py[q] = pix_y(pj[q], vp);
+    px[q] = inside[q] ? sx[pi[q]] : 0.0f;
+    py[q] = inside[q] ? sx[f.W + pj[q]] : 0.0f;
     // start from the small triangles already resolved by k_setup (in the
     // narrow form their id is not kept: -1, which no equal key displaces)
     best_key[q] = 0xFFFFFFFFu;
@@ -722,10 +737,26 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
       if (t < 0) continue;
       const float4 r0 = lds_rec[3 * m + 0], r1 = lds_rec[3 * m + 1],
                    r2 = lds_rec[3 * m + 2];
+      // y * b_k is one rounded product, shared by pixels of the same row
+      const float yb0 = py[0] * r0.y, yb1 = py[0] * r1.y, yb2 = py[0] * r2.y;
 #pragma unroll
       for (int q = 0; q < PPL; ++q) {
-        const float e0 = edge(r0, px[q], py[q]), e1 = edge(r1, px[q], py[q]),
-                    e2 = edge(r2, px[q], py[q]);
+        float e0, e1, e2;
+        if (kOneRow) {
+          e0 = px[q] * r0.x;
+          e0 = e0 + yb0;
+          e0 = e0 + r0.z;
+          e1 = px[q] * r1.x;
+          e1 = e1 + yb1;
+          e1 = e1 + r1.z;
+          e2 = px[q] * r2.x;
+          e2 = e2 + yb2;
+          e2 = e2 + r2.z;
+        } else {
+          e0 = edge(r0, px[q], py[q]);
+          e1 = edge(r1, px[q], py[q]);
+          e2 = edge(r2, px[q], py[q]);
+        }
         if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
           const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
           if (key < best_key[q] || (key == best_key[q] && t < best_id[q])) {
@@ -1114,6 +1145,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.offsets = offsets;
   a.bins = bins;
   a.keys = keys;
+  a.samples = samples;
   a.work_counter = misc + 1;
   a.chunk = 1;
   a.xcd_scenes = 0;
