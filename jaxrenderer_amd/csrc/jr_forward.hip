@@ -284,7 +284,8 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     float4* __restrict__ rec, Rect* __restrict__ rects,
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
     int32_t* __restrict__ wide, int32_t* __restrict__ wide_count,
-    const float* __restrict__ samples, uint32_t* __restrict__ zkeys) {
+    const float* __restrict__ samples, uint32_t* __restrict__ zkeys,
+    uint32_t* __restrict__ masks) {
   __shared__ SmallStage sm;
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < in.n_tris;
@@ -332,12 +333,19 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     if (n_t > kWideTiles) {
       wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
     } else {
+      // count the tiles that may hold a covered pixel; remember which ones
+      // (bit q: tile (tx0 + q / h, ty0 + q % h)) for k_bin
       int32_t* cnt = counts + (int64_t)s * f.T;
       const float* sxs = samples + (int64_t)s * (f.W + f.H);
+      uint32_t mask = 0u;
+      int q = 0;
       for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
-        for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
-          if (tile_may_cover(r0, r1, r2, sane, sc.viewport, sxs, f, tx, ty))
+        for (int32_t ty = r.ty0; ty <= r.ty1; ++ty, ++q)
+          if (tile_may_cover(r0, r1, r2, sane, sc.viewport, sxs, f, tx, ty)) {
             atomicAdd(cnt + tx * f.tiles_y + ty, 1);
+            mask |= 1u << q;
+          }
+      masks[t] = mask;
     }
   }
   int32_t incl = n_small;
@@ -499,27 +507,25 @@ __global__ __launch_bounds__(256) void k_bin(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     const float4* __restrict__ rec, const Rect* __restrict__ rects,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ bins, const float* __restrict__ samples) {
+    int32_t* __restrict__ bins, const uint32_t* __restrict__ masks) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= in.n_tris) return;
   const Rect r = rects[t];
   if (r.tx0 > r.tx1) return;
   if ((int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1) > kWideTiles) return;  // k_wide
-  const uint32_t fl = __float_as_uint(rec[4 * (int64_t)t + 3].y);
+  uint32_t mask = masks[t];  // the tiles k_setup counted
+  if (mask == 0u) return;
   const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
-  const float* vp = scenes[s].viewport;
-  const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
-               r2 = rec[4 * (int64_t)t + 2];
-  const bool sane = (fl & kSane) != 0;
   const int64_t tbase = (int64_t)s * f.T;
-  const float* sxs = samples + (int64_t)s * (f.W + f.H);
-  for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
-    for (int32_t ty = r.ty0; ty <= r.ty1; ++ty)
-      if (tile_may_cover(r0, r1, r2, sane, vp, sxs, f, tx, ty)) {
-        const int64_t gt = tbase + tx * f.tiles_y + ty;
-        const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
-        if (pos < f.capacity) bins[pos] = t;
-      }
+  const int32_t h = r.ty1 - r.ty0 + 1;
+  while (mask) {
+    const int q = __ffs(mask) - 1;
+    mask &= mask - 1u;
+    const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
+    const int64_t gt = tbase + tx * f.tiles_y + ty;
+    const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
+    if (pos < f.capacity) bins[pos] = t;
+  }
 }
 
 // Stage 4: per-tile visibility, shading and merge.
@@ -884,8 +890,8 @@ constexpr int64_t kAlign = 256;
 int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
-  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, samples, bins,
-      total;
+  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, samples, masks,
+      bins, total;
   int64_t n_tiles, nblk, n_pix;
 };
 
@@ -907,6 +913,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.keys = o;       o += align_up(8 * L.n_pix);
   L.wide = o;       o += align_up(4 * (int64_t)std::max(n_tris, 1));
   L.samples = o;    o += align_up(4 * (int64_t)(fr->width + fr->height) * fr->n_scenes);
+  L.masks = o;      o += align_up(4 * (int64_t)std::max(n_tris, 1));
   L.bins = o;       o += align_up(4 * std::max<int64_t>(fr->bin_capacity, 1));
   L.total = o;
   return L;
@@ -1096,6 +1103,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   uint64_t* keys = reinterpret_cast<uint64_t*>(ws + L.keys);
   int32_t* wide = reinterpret_cast<int32_t*>(ws + L.wide);
   float* samples = reinterpret_cast<float*>(ws + L.samples);
+  uint32_t* masks = reinterpret_cast<uint32_t*>(ws + L.masks);
   int32_t* wide_count = misc + 2;
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
@@ -1115,7 +1123,8 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   if (n_tris > 0)
     hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
                        dim3(kSetupThreads), 0, st, *in,
-                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples, zkeys);
+                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples, zkeys,
+                       masks);
   timer.mark(2);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
@@ -1130,7 +1139,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   timer.mark(4);
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rec, rects, offsets, cursor, bins, samples);
+                       scenes, f, rec, rects, offsets, cursor, bins, masks);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
                        rects, wide, wide_count, counts, offsets, cursor, bins, samples);
   }
