@@ -95,24 +95,33 @@ struct TriSetup {
   bool keep, front, sane;
 };
 
-JR_DEV void tri_setup(const jr_inputs& in, const jr_scene& sc, int32_t t, TriSetup& o) {
+// Clip coordinates of face row t's three slots (the vertex stage's positions).
+JR_DEV void tri_corners(const jr_inputs& in, const jr_scene& sc, int32_t t, float clip[3][4]) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int32_t slot = in.faces[3 * (int64_t)t + k];
     if (in.clip) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) o.clip[k][q] = in.clip[4 * (int64_t)slot + q];
+      for (int q = 0; q < 4; ++q) clip[k][q] = in.clip[4 * (int64_t)slot + q];
     } else {
-      position_clip(in, sc.world_to_clip, slot, o.clip[k]);
+      position_clip(in, sc.world_to_clip, slot, clip[k]);
     }
   }
+}
+
+// The [x, y, w] matrix of the corners and its determinant.
+JR_DEV float tri_det(const float clip[3][4], float M[9]) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    o.M[3 * k + 0] = o.clip[k][0];
-    o.M[3 * k + 1] = o.clip[k][1];
-    o.M[3 * k + 2] = o.clip[k][3];
+    M[3 * k + 0] = clip[k][0];
+    M[3 * k + 1] = clip[k][1];
+    M[3 * k + 2] = clip[k][3];
   }
-  o.det = det3(o.M);
+  return det3(M);
+}
+
+// Keep / front-facing flags and the inverse, given the corners and det.
+JR_DEV void tri_finish(TriSetup& o) {
   o.keep = fabsf(o.det) > 1e-6f;
   o.front = o.det >= 0.0f;
   inv3(o.M, o.Mi);
@@ -120,6 +129,12 @@ JR_DEV void tri_setup(const jr_inputs& in, const jr_scene& sc, int32_t t, TriSet
 #pragma unroll
   for (int i = 0; i < 9; ++i) sane = sane && (fabsf(o.Mi[i]) < 1e18f);
   o.sane = sane;
+}
+
+JR_DEV void tri_setup(const jr_inputs& in, const jr_scene& sc, int32_t t, TriSetup& o) {
+  tri_corners(in, sc, t, o.clip);
+  o.det = tri_det(o.clip, o.M);
+  tri_finish(o);
 }
 
 JR_DEV float4 rec_col(const TriSetup& o, int k) {

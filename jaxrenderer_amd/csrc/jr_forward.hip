@@ -279,6 +279,18 @@ struct SmallStage {
   int32_t off[kSetupThreads];  // exclusive prefix sum of the areas per wavefront
 };
 
+// A triangle handed from the first half of k_setup to the second: its clip
+// corners (x, y, z, w) and (id, scene, det bits).
+struct SetupItem {
+  float4 c[3];
+  int4 meta;
+};
+
+union SetupStage {
+  SetupItem item[kSetupThreads];
+  SmallStage sm;
+};
+
 __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     float4* __restrict__ rec, Rect* __restrict__ rects,
@@ -286,17 +298,85 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     int32_t* __restrict__ wide, int32_t* __restrict__ wide_count,
     const float* __restrict__ samples, uint32_t* __restrict__ zkeys,
     uint32_t* __restrict__ masks) {
-  __shared__ SmallStage sm;
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = t < in.n_tris;
+  __shared__ SetupStage stage;
+  __shared__ int32_t wave_live[kSetupThreads / 64];
+  SmallStage& sm = stage.sm;
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
+
+  // First half, every triangle: corners, det, keep / front.  Only a kept
+  // front-facing triangle can win a pixel, and besides those only a scene's
+  // first triangle (argmin's choice when nothing covers a pixel) has its
+  // edge coefficients read (the others only their flags), so only these
+  // "live" triangles go on to the inverse and the pixel range.  They are packed to the front of the workgroup, so the
+  // second half runs on about half as many wavefronts.
+  {
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = false;
+    float clip[3][4];
+    float det = 0.0f;
+    int32_t s = 0;
+    if (t < in.n_tris) {
+      s = scene_of_tri_wave(scenes, f.n_scenes, t);
+      const jr_scene& sc = scenes[s];
+      tri_corners(in, sc, t, clip);
+      float M[9];
+      det = tri_det(clip, M);
+      const bool keep = fabsf(det) > 1e-6f, front = det >= 0.0f;
+      live = (keep && front) || t == sc.tri_begin;
+      if (!live) {
+        // no range, and the flags alone (read by the every-triangle scan of an
+        // overflowing tile, which skips such a triangle)
+        rects[t] = Rect{1, 0, 1, 0};
+        const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u);
+        rec[4 * (int64_t)t + 3] = make_float4(det, __uint_as_float(flags), 0.0f, 0.0f);
+      }
+    }
+    const uint64_t bal = __ballot(live);
+    if (lane == 0) wave_live[threadIdx.x >> 6] = __popcll(bal);
+    __syncthreads();
+    int32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) pos += wave_live[w];
+    if (live) {
+      SetupItem& it = stage.item[pos];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        it.c[k] = make_float4(clip[k][0], clip[k][1], clip[k][2], clip[k][3]);
+      it.meta = make_int4(t, s, (int32_t)__float_as_uint(det), 0);
+    }
+    __syncthreads();
+  }
+  int32_t n_live = 0;
+#pragma unroll
+  for (int w = 0; w < kSetupThreads / 64; ++w) n_live += wave_live[w];
+  const bool valid = (int32_t)threadIdx.x < n_live;
+  TriSetup ts;
+  int32_t t = 0, s = 0;
+  if (valid) {
+    const SetupItem& it = stage.item[threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float4 c = it.c[k];
+      ts.clip[k][0] = c.x;
+      ts.clip[k][1] = c.y;
+      ts.clip[k][2] = c.z;
+      ts.clip[k][3] = c.w;
+    }
+    t = it.meta.x;
+    s = it.meta.y;
+    ts.det = __uint_as_float((uint32_t)it.meta.z);
+  }
+  __syncthreads();  // the items' storage becomes the small-triangle stage
+
+  // Second half, live triangles: inverse, records, pixel range; small
+  // triangles are resolved into the key buffer here, the others counted into
+  // the tiles they may cover.
   int32_t n_small = 0;
   if (valid) {
-    const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
     const jr_scene& sc = scenes[s];
-    TriSetup ts;
-    tri_setup(in, sc, t, ts);
+    tri_det(ts.clip, ts.M);
+    tri_finish(ts);
     const bool keep = ts.keep, front = ts.front, sane = ts.sane;
     const float4 r0 = rec_col(ts, 0);
     const float4 r1 = rec_col(ts, 1);

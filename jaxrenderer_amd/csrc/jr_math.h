@@ -79,6 +79,9 @@ JR_DEV float det3(const float a[9]) {
 // when |pivot| >= FLT_MIN), then the permuted identity solved with the
 // reference-BLAS strsm loops (lower unit, then upper), skipping zero entries.
 JR_DEV void inv3(const float A[9], float X[9]) {
+  // every index below is a compile-time constant (the pivot row is chosen by
+  // conditional swaps with each candidate row), so the matrix stays in
+  // registers
   float a[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) a[i] = A[i];
@@ -87,26 +90,31 @@ JR_DEV void inv3(const float A[9], float X[9]) {
   for (int j = 0; j < 3; ++j) {
     int p = j;
     float amax = fabsf(a[3 * j + j]);
+    float apj = a[3 * j + j];
 #pragma unroll
     for (int i = j + 1; i < 3; ++i) {
       const float v = fabsf(a[3 * i + j]);
       if (v > amax) {
         amax = v;
         p = i;
+        apj = a[3 * i + j];
       }
     }
-    if (a[3 * p + j] != 0.0f) {
-      if (p != j) {
+    const bool nz = apj != 0.0f;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float t = a[3 * p + k];
-          a[3 * p + k] = a[3 * j + k];
-          a[3 * j + k] = t;
-        }
-        const int tp = perm[p];
-        perm[p] = perm[j];
-        perm[j] = tp;
+    for (int i = j + 1; i < 3; ++i) {
+      const bool sw = nz && p == i;  // at most one candidate row swaps with row j
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float t = a[3 * i + k], u = a[3 * j + k];
+        a[3 * i + k] = sw ? u : t;
+        a[3 * j + k] = sw ? t : u;
       }
+      const int tp = perm[i], up = perm[j];
+      perm[i] = sw ? up : tp;
+      perm[j] = sw ? tp : up;
+    }
+    if (nz) {
       const float piv = a[3 * j + j];
       if (fabsf(piv) >= 1.17549435e-38f) {
         const float r = 1.0f / piv;
