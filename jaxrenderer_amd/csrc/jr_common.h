@@ -84,6 +84,28 @@ JR_DEV int32_t scene_of_tri_wave(const jr_scene* __restrict__ scenes, int32_t n,
   return scene_of_tri(scenes, n, t);
 }
 
+// scene_of_tri for the wavefront of triangles [t0, t0 + 64), t0 = t & ~63,
+// from a table holding the scene of every wavefront's first triangle
+// (k_clear fills it): one scalar load, then a wavefront-uniform walk over the
+// scenes that begin inside the wavefront's range.  Call from converged code
+// of a kernel whose wavefronts hold 64 consecutive triangle ids.
+JR_DEV int32_t scene_of_wave(const jr_scene* __restrict__ scenes, int32_t n,
+                             const int32_t* __restrict__ wave_scene, int32_t t,
+                             int32_t n_tris) {
+  const int32_t t0 = __builtin_amdgcn_readfirstlane(t & ~63);
+  int32_t su = __builtin_amdgcn_readfirstlane(wave_scene[t0 >> 6]);
+  su = min(max(su, 0), n - 1);
+  const int32_t tl = min(t0 + 63, n_tris - 1);
+  int32_t s = su;
+  while (su + 1 < n) {
+    const int32_t nb = scenes[su + 1].tri_begin;
+    if (nb > tl) break;
+    ++su;
+    if (t >= nb) s = su;
+  }
+  return s;
+}
+
 // PerPrimitive.create (renderer/pipeline.py:76-113) for face row t: clip
 // coordinates of its three slots, the [x, y, w] matrix, its determinant,
 // keep / front-facing flags and the inverse.

@@ -218,6 +218,14 @@ JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
 // whole wavefront (one lane per tile) instead of by their own setup thread.
 constexpr int32_t kWideTiles = 16;
 
+// Per-triangle binning word written by k_setup: kMaskNone (culled, small,
+// or no tile may hold a covered pixel: nothing to bin and no tile range
+// stored), kMaskWide (binned by k_wide), else bit q set for each counted tile
+// (tx0 + q / h, ty0 + q % h) of the stored range, h = ty1 - ty0 + 1 (at most
+// kWideTiles bits).
+constexpr uint32_t kMaskNone = 0u;
+constexpr uint32_t kMaskWide = ~0u;
+
 JR_DEV bool has_nan_column(const float4& r) {
   return (r.x != r.x) || (r.y != r.y) || (r.z != r.z);
 }
@@ -229,9 +237,23 @@ JR_DEV bool has_nan_column(const float4& r) {
 __global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __restrict__ p64,
                         int64_t n64, uint32_t* __restrict__ k32, int64_t nk32,
                         const jr_scene* __restrict__ scenes, Frame f,
-                        float* __restrict__ samples) {
+                        float* __restrict__ samples, int32_t* __restrict__ wave_scene,
+                        int32_t n_tris) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // wave_scene[w] = the scene of triangle 64 w (the last scene whose
+  // tri_begin <= 64 w): one wavefront per scene writes the entries whose
+  // first triangle lies in [tri_begin of the scene, tri_begin of the next)
+  {
+    const int32_t n_w = (n_tris + 63) >> 6;
+    const int lane = threadIdx.x & 63;
+    for (int64_t s = first >> 6; s < f.n_scenes; s += stride >> 6) {
+      const int32_t lo = s == 0 ? 0 : max(scenes[s].tri_begin, 0);
+      const int32_t hi = s + 1 < f.n_scenes ? max(scenes[s + 1].tri_begin, 0) : n_tris;
+      const int32_t w1 = min((hi + 63) >> 6, n_w);
+      for (int32_t w = ((lo + 63) >> 6) + lane; w < w1; w += 64) wave_scene[w] = (int32_t)s;
+    }
+  }
   for (int64_t i = first; i < n32; i += stride) p32[i] = 0;
   for (int64_t i = first; i < n64; i += stride) p64[i] = ~0ull;
   for (int64_t i = first; i < nk32; i += stride) k32[i] = ~0u;
@@ -297,7 +319,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
     int32_t* __restrict__ wide, int32_t* __restrict__ wide_count,
     const float* __restrict__ samples, uint32_t* __restrict__ zkeys,
-    uint32_t* __restrict__ masks) {
+    uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene) {
   __shared__ SetupStage stage;
   __shared__ int32_t wave_live[kSetupThreads / 64];
   SmallStage& sm = stage.sm;
@@ -305,32 +327,28 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
   const int wbase = threadIdx.x & ~63;
 
   // First half, every triangle: corners, det, keep / front.  Only a kept
-  // front-facing triangle can win a pixel, and besides those only a scene's
+  // front-facing triangle can win a pixel; besides those only a scene's
   // first triangle (argmin's choice when nothing covers a pixel) has its
-  // edge coefficients read (the others only their flags), so only these
-  // "live" triangles go on to the inverse and the pixel range.  They are packed to the front of the workgroup, so the
-  // second half runs on about half as many wavefronts.
+  // record read.  Only these "live" triangles go on to the inverse and the
+  // pixel range; the others just record that they have nothing to bin.  The
+  // live ones are packed to the front of the workgroup, so the second half
+  // runs on about half as many wavefronts.
   {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = false;
     float clip[3][4];
     float det = 0.0f;
     int32_t s = 0;
+    if (__builtin_amdgcn_readfirstlane(t & ~63) < in.n_tris)
+      s = scene_of_wave(scenes, f.n_scenes, wave_scene, t, in.n_tris);
     if (t < in.n_tris) {
-      s = scene_of_tri_wave(scenes, f.n_scenes, t);
       const jr_scene& sc = scenes[s];
       tri_corners(in, sc, t, clip);
       float M[9];
       det = tri_det(clip, M);
       const bool keep = fabsf(det) > 1e-6f, front = det >= 0.0f;
       live = (keep && front) || t == sc.tri_begin;
-      if (!live) {
-        // no range, and the flags alone (read by the every-triangle scan of an
-        // overflowing tile, which skips such a triangle)
-        rects[t] = Rect{1, 0, 1, 0};
-        const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u);
-        rec[4 * (int64_t)t + 3] = make_float4(det, __uint_as_float(flags), 0.0f, 0.0f);
-      }
+      if (!live) masks[t] = kMaskNone;  // nothing to bin, nothing else read
     }
     const uint64_t bal = __ballot(live);
     if (lane == 0) wave_live[threadIdx.x >> 6] = __popcll(bal);
@@ -387,10 +405,6 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     const bool nan_col = has_nan_column(r0) || has_nan_column(r1) || has_nan_column(r2);
     if (keep && front && !nan_col) px = pixel_range(r0, r1, r2, sane, sc.viewport, f);
     const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
-    rec[4 * (int64_t)t + 0] = r0;
-    rec[4 * (int64_t)t + 1] = r1;
-    rec[4 * (int64_t)t + 2] = r2;
-    rec[4 * (int64_t)t + 3] = make_float4(ts.det, __uint_as_float(flags), 0.0f, 0.0f);
     const int64_t area = (px.tx0 > px.tx1)
         ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
     // the narrow form reconstructs a small winner's depth from its key, so its
@@ -398,8 +412,16 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     const bool small = area > 0 && area <= kSmallPixels &&
                        (!f.narrow || depth_bounded(r0, r1, r2, px, sc.viewport,
                                                    samples + (int64_t)s * (f.W + f.H), f));
+    // the record is read for binned triangles, for pixel winners and for the
+    // scene's first triangle; the narrow form finishes its small winners
+    // from their keys alone
+    if (!(f.narrow && small) || t == sc.tri_begin) {
+      rec[4 * (int64_t)t + 0] = r0;
+      rec[4 * (int64_t)t + 1] = r1;
+      rec[4 * (int64_t)t + 2] = r2;
+      rec[4 * (int64_t)t + 3] = make_float4(ts.det, __uint_as_float(flags), 0.0f, 0.0f);
+    }
     if (area > 0 && !small) r = tiles_of(px, f);
-    rects[t] = r;
     if (small) {
       n_small = (int32_t)area;
       sm.r[0][threadIdx.x] = r0;
@@ -410,8 +432,10 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     }
     const int64_t n_t =
         (r.tx0 > r.tx1) ? 0 : (int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
+    if (n_t > 0) rects[t] = r;
     if (n_t > kWideTiles) {
       wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
+      masks[t] = kMaskWide;
     } else {
       // count the tiles that may hold a covered pixel; remember which ones
       // (bit q: tile (tx0 + q / h, ty0 + q % h)) for k_bin
@@ -587,15 +611,15 @@ __global__ __launch_bounds__(256) void k_bin(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     const float4* __restrict__ rec, const Rect* __restrict__ rects,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ bins, const uint32_t* __restrict__ masks) {
+    int32_t* __restrict__ bins, const uint32_t* __restrict__ masks,
+    const int32_t* __restrict__ wave_scene) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (__builtin_amdgcn_readfirstlane(t & ~63) >= in.n_tris) return;
+  const int32_t s = scene_of_wave(scenes, f.n_scenes, wave_scene, t, in.n_tris);
   if (t >= in.n_tris) return;
-  const Rect r = rects[t];
-  if (r.tx0 > r.tx1) return;
-  if ((int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1) > kWideTiles) return;  // k_wide
   uint32_t mask = masks[t];  // the tiles k_setup counted
-  if (mask == 0u) return;
-  const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
+  if (mask == kMaskNone || mask == kMaskWide) return;  // nothing, or k_wide's
+  const Rect r = rects[t];
   const int64_t tbase = (int64_t)s * f.T;
   const int32_t h = r.ty1 - r.ty0 + 1;
   while (mask) {
@@ -606,6 +630,19 @@ __global__ __launch_bounds__(256) void k_bin(
     const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
     if (pos < f.capacity) bins[pos] = t;
   }
+}
+
+// True when k_setup counted tile (tx, ty) for triangle t (any tile of its
+// range for a wide triangle).
+JR_DEV bool lists_tile(const Rect* __restrict__ rects, const uint32_t* __restrict__ masks,
+                       int32_t t, int32_t tx, int32_t ty) {
+  const uint32_t m = masks[t];
+  if (m == kMaskNone) return false;
+  const Rect r = rects[t];
+  if (tx < r.tx0 || tx > r.tx1 || ty < r.ty0 || ty > r.ty1) return false;
+  if (m == kMaskWide) return true;
+  const int32_t q = (tx - r.tx0) * (r.ty1 - r.ty0 + 1) + (ty - r.ty0);
+  return ((m >> q) & 1u) != 0u;
 }
 
 // Stage 4: per-tile visibility, shading and merge.
@@ -620,6 +657,8 @@ struct RasterArgs {
   const int32_t* bins;
   const uint64_t* keys;
   const float* samples;  // per scene: pix_x(i) for i < W, then pix_y(j)
+  const Rect* rects;     // per-triangle tile ranges and binning words (k_setup)
+  const uint32_t* masks;
   int32_t* work_counter;
   int32_t chunk;  // tiles claimed per work-queue step (persistent form)
   int32_t xcd_scenes;  // one-group-per-tile form: scenes kept on one XCD each
@@ -807,15 +846,17 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
     const int32_t n = min(NT, n_items - base);
     __syncthreads();
     if (tid < n) {
+      // binned triangles are kept and front-facing; the scan of an
+      // overflowing tile takes the scene's triangles that list this tile (the
+      // small ones are already in the key buffer, the culled ones never win)
       const int32_t t = overflow ? (sc.tri_begin + base + tid) : a.bins[start + base + tid];
-      const float4 r0 = a.rec[4 * (int64_t)t], r1 = a.rec[4 * (int64_t)t + 1],
-                   r2 = a.rec[4 * (int64_t)t + 2];
-      const uint32_t fl = __float_as_uint(a.rec[4 * (int64_t)t + 3].y);
-      lds_rec[3 * tid + 0] = r0;
-      lds_rec[3 * tid + 1] = r1;
-      lds_rec[3 * tid + 2] = r2;
-      // triangles that are not kept or not front-facing never win the chooser
-      lds_id[tid] = ((fl & (kKeep | kFront)) == (kKeep | kFront)) ? t : -1;
+      const bool use = !overflow || lists_tile(a.rects, a.masks, t, tx, ty);
+      if (use) {
+        lds_rec[3 * tid + 0] = a.rec[4 * (int64_t)t];
+        lds_rec[3 * tid + 1] = a.rec[4 * (int64_t)t + 1];
+        lds_rec[3 * tid + 2] = a.rec[4 * (int64_t)t + 2];
+      }
+      lds_id[tid] = use ? t : -1;
     }
     __syncthreads();
     for (int m = 0; m < n; ++m) {
@@ -971,7 +1012,7 @@ int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
   int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, samples, masks,
-      bins, total;
+      wave_scene, bins, total;
   int64_t n_tiles, nblk, n_pix;
 };
 
@@ -994,6 +1035,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.wide = o;       o += align_up(4 * (int64_t)std::max(n_tris, 1));
   L.samples = o;    o += align_up(4 * (int64_t)(fr->width + fr->height) * fr->n_scenes);
   L.masks = o;      o += align_up(4 * (int64_t)std::max(n_tris, 1));
+  L.wave_scene = o; o += align_up(4 * (((int64_t)std::max(n_tris, 1) + 63) / 64));
   L.bins = o;       o += align_up(4 * std::max<int64_t>(fr->bin_capacity, 1));
   L.total = o;
   return L;
@@ -1184,6 +1226,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   int32_t* wide = reinterpret_cast<int32_t*>(ws + L.wide);
   float* samples = reinterpret_cast<float*>(ws + L.samples);
   uint32_t* masks = reinterpret_cast<uint32_t*>(ws + L.masks);
+  int32_t* wave_scene = reinterpret_cast<int32_t*>(ws + L.wave_scene);
   int32_t* wide_count = misc + 2;
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
@@ -1197,14 +1240,15 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   uint32_t* zkeys = reinterpret_cast<uint32_t*>(out->zbuffer);
   hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<int64_t>((nclr + 255) / 256, 8192)),
                      dim3(256), 0, st, counts, nz, keys, f.narrow ? (int64_t)0 : L.n_pix,
-                     zkeys, f.narrow ? L.n_pix : (int64_t)0, scenes, f, samples);
+                     zkeys, f.narrow ? L.n_pix : (int64_t)0, scenes, f, samples, wave_scene,
+                     in->n_tris);
   timer.mark(1);
   const int n_tris = in->n_tris;
   if (n_tris > 0)
     hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
                        dim3(kSetupThreads), 0, st, *in,
                        scenes, f, rec, rects, counts, keys, wide, wide_count, samples, zkeys,
-                       masks);
+                       masks, wave_scene);
   timer.mark(2);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
@@ -1219,7 +1263,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   timer.mark(4);
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rec, rects, offsets, cursor, bins, masks);
+                       scenes, f, rec, rects, offsets, cursor, bins, masks, wave_scene);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
                        rects, wide, wide_count, counts, offsets, cursor, bins, samples);
   }
@@ -1233,6 +1277,8 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   a.counts = counts;
   a.offsets = offsets;
   a.bins = bins;
+  a.rects = rects;
+  a.masks = masks;
   a.keys = keys;
   a.samples = samples;
   a.work_counter = misc + 1;

@@ -145,3 +145,32 @@ def test_depth_only_offset_pass(seed, tile, ppl):
     got = _plan.run_plan(shadow, tile=tile, pixels_per_lane=ppl)
     torch.cuda.synchronize()
     assert_parity(got["shadow"], ref["shadow"], label=f"c3shadow{seed}")
+
+
+@pytest.mark.parametrize("shader", [DepthShader, PhongReflectionShadowTextureShader],
+                         ids=lambda s: s.__name__)
+def test_ragged_and_empty_scenes(shader):
+    """One call whose scenes own 0, 1, 3, 63, 64, 65, ... triangles, empty
+    scenes among them (first and last included): each triangle belongs to
+    the last scene whose range begins at or before it, wavefronts of 64
+    triangles span several scenes, and an empty scene draws nothing."""
+    sizes = [0, 3, 64, 0, 0, 130, 1, 63, 65, 0, 200, 171, 0]
+
+    def calls():
+        out = scenes.soup_call(shader, 9, 40, 30, sum(sizes))
+        for c in out:
+            base = c.scenes[0]
+            c.scenes = []
+            begin = 0
+            for n in sizes:
+                sc = type(base).from_buffer_copy(base)
+                sc.tri_begin, sc.tri_end = begin, begin + n
+                begin += n
+                c.scenes.append(sc)
+        return out
+
+    got, _ = _check(calls, f"ragged/{shader.__name__}", tile=(8, 8))
+    pid = got["main"]["prim_id"].cpu().numpy()
+    for s, n in enumerate(sizes):
+        if n == 0:
+            assert (pid[s] == -1).all()
