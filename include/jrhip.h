@@ -151,7 +151,8 @@ typedef struct jr_frame {
   int32_t tile_h;         /* pixels per tile along height (y) */
   int32_t persistent;     /* 1: one persistent raster launch draining a work queue */
   int32_t tiles_per_group; /* otherwise: consecutive tiles each workgroup rasterises
-                              (0 or 1: one) */
+                              (0 or 1: one); with one wavefront per tile and at most 4,
+                              side by side (a wavefront each), else one after another */
   int32_t pixels_per_lane; /* pixels per raster thread: 0/1, or 4 with 16x16 or 8x32
                               tiles (one wavefront per tile) */
   int32_t pad_i;

@@ -5,8 +5,9 @@ launch shapes of `jr_forward` on the live data: the screen-tile shape (which
 also sets the workgroup: one lane per pixel, or per 4 pixels for the
 256-pixel tiles), the launch form of the raster stage (a persistent
 work-queue grid, or a grid of workgroups taking 1, 2 or 4 consecutive tiles
-each), and, for each, the bin capacity sized from the binned triangle-tile
-pairs that shape produces.
+each: one after another, or, at 4 pixels per lane, one wavefront per tile side
+by side), and, for each, the bin capacity sized from the binned
+triangle-tile pairs that shape produces.
 Every candidate must reproduce the first candidate's depth buffer bit for bit
 (the binning is exact, so the tile shape cannot change the image); a
 mismatch raises.

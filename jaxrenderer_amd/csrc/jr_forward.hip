@@ -787,15 +787,31 @@ JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int
 // One screen tile of TW x TH pixels with TW * TH / PPL threads: thread tid
 // owns the pixels tid + q * (TW * TH / PPL), q < PPL.  The tile's triangle
 // list is staged through LDS once and tested against all of a thread's pixels.
-template <int SHADER, int TW, int TH, int PPL>
+// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
+// each wavefront of the workgroup works on a tile of its own, a wavefront
+// barrier (a wavefront's LDS operations complete in issue order, so keeping
+// the compiler from moving memory operations across the point is enough).
+template <bool WAVE>
+JR_DEV void tile_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int SHADER, int TW, int TH, int PPL, bool WAVE = false>
 JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
                         int32_t* lds_id) {
   constexpr int NT = TW * TH / PPL;
+  static_assert(!WAVE || NT == 64, "a wavefront-owned tile has one lane per 4 pixels");
   const Frame& f = a.f;
   const int32_t s = (int32_t)(gt / f.T);
   const int32_t tile = (int32_t)(gt - (int64_t)s * f.T);
   const int32_t tx = tile / f.tiles_y, ty = tile - tx * f.tiles_y;
-  const int tid = threadIdx.x;
+  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const jr_scene& sc = a.scenes[s];
   const float* vp = sc.viewport;
   const float vp22 = vp[10], vp23 = vp[11];
@@ -844,7 +860,7 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
   const int32_t n_items = overflow ? (sc.tri_end - sc.tri_begin) : cnt;
   for (int32_t base = 0; base < n_items; base += NT) {
     const int32_t n = min(NT, n_items - base);
-    __syncthreads();
+    tile_sync<WAVE>();
     if (tid < n) {
       // binned triangles are kept and front-facing; the scan of an
       // overflowing tile takes the scene's triangles that list this tile (the
@@ -858,7 +874,7 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
       }
       lds_id[tid] = use ? t : -1;
     }
-    __syncthreads();
+    tile_sync<WAVE>();
     for (int m = 0; m < n; ++m) {
       const int32_t t = lds_id[m];
       if (t < 0) continue;
@@ -926,6 +942,25 @@ __global__ __launch_bounds__(TW* TH / PPL) void k_raster(RasterArgs a) {
     if (gt < 0) return;
   }
   raster_tile<SHADER, TW, TH, PPL>(a, gt, lds_rec, lds_id);
+}
+
+// Several consecutive 256-pixel tiles per workgroup, one per wavefront, side
+// by side (four pixels per lane).
+template <int SHADER, int TW, int TH, int PPL>
+__global__ __launch_bounds__(256) void k_raster_waves(RasterArgs a) {
+  __shared__ float4 lds_rec[4][3 * 64];
+  __shared__ int32_t lds_id[4][64];
+  const int w = threadIdx.x >> 6;
+  const int64_t b = blockIdx.x;
+  int64_t gt;
+  if (a.xcd_scenes) {
+    gt = xcd_item(a, (int32_t)(b % kXcds), (b / kXcds) * a.tiles_per_group + w);
+    if (gt < 0) return;
+  } else {
+    gt = b * a.tiles_per_group + w;
+    if (gt >= (int64_t)a.f.n_scenes * a.f.T) return;
+  }
+  raster_tile<SHADER, TW, TH, PPL, true>(a, gt, lds_rec[w], lds_id[w]);
 }
 
 // Several consecutive tiles per workgroup (a separate kernel: the loop raises
@@ -1089,12 +1124,20 @@ void launch_raster_t(const RasterArgs& a, int64_t n_items, bool persistent,
       const int64_t per_xcd = (int64_t)((a.f.n_scenes + kXcds - 1) / kXcds) * a.f.T;
       grid = kXcds * ((per_xcd + tpg - 1) / tpg);
     }
-    if (tpg == 1)
+    if (tpg == 1) {
       hipLaunchKernelGGL((k_raster<SHADER, TW, TH, PPL>), dim3((unsigned)grid), dim3(NT), 0,
                          st, b);
-    else
-      hipLaunchKernelGGL((k_raster_multi<SHADER, TW, TH, PPL>), dim3((unsigned)grid),
-                         dim3(NT), 0, st, b);
+      return;
+    }
+    if constexpr (NT == 64) {
+      if (tpg <= 4) {
+        hipLaunchKernelGGL((k_raster_waves<SHADER, TW, TH, PPL>), dim3((unsigned)grid),
+                           dim3((unsigned)(64 * tpg)), 0, st, b);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_raster_multi<SHADER, TW, TH, PPL>), dim3((unsigned)grid),
+                       dim3(NT), 0, st, b);
   }
 }
 

@@ -107,16 +107,18 @@ def test_tiny_and_pixel_aligned_triangles(seed, tile):
     _check(lambda: scenes.tiny_soup_call(seed, shader=shader), f"tiny{seed}", tile=tile)
 
 
-@pytest.mark.parametrize("tiles_per_group", [1, 3, 4])
-def test_many_scenes_xcd_mapping_and_tiles_per_group(tiles_per_group):
+@pytest.mark.parametrize("tiles_per_group,ppl", [(1, 1), (3, 1), (4, 1), (2, 4), (4, 4), (6, 4)],
+                         ids=str)
+def test_many_scenes_xcd_mapping_and_tiles_per_group(tiles_per_group, ppl):
     """70 scenes (>= 64: the raster keeps each scene on one XCD) at 40x30 with
-    16x16 tiles (partial edge tiles) and several tiles per workgroup: every
-    (scene, tile) is rendered exactly once and matches the oracle."""
+    16x16 tiles (partial edge tiles) and several tiles per workgroup (with four
+    pixels per lane and at most 4 tiles, one wavefront per tile side by side):
+    every (scene, tile) is rendered exactly once and matches the oracle."""
     from jaxrenderer_amd import batch, workloads
 
     scs, sm = workloads.config3(n_scenes=70, seed=12, width=40, height=30, shadow_size=24)
-    _check(lambda: batch.plan_batch(scs, 40, 30, sm), f"xcd{tiles_per_group}", tile=(16, 16),
-           tiles_per_group=tiles_per_group)
+    _check(lambda: batch.plan_batch(scs, 40, 30, sm), f"xcd{tiles_per_group}x{ppl}",
+           tile=(16, 16), tiles_per_group=tiles_per_group, pixels_per_lane=ppl)
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
