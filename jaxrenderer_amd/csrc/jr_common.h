@@ -52,6 +52,10 @@ JR_DEV float win_depth(const float4& r0, const float4& r1, const float4& r2,
 
 struct Frame {
   int32_t W, H, TW, TH, tiles_x, tiles_y, T;
+  int32_t tw_sh, th_sh;  // log2 TW, log2 TH
+  // n / T and n / tiles_y for 0 <= n < 2^31 as (n * m) >> sh (udiv31)
+  uint32_t T_m, ty_m;
+  int32_t T_sh, ty_sh;
   int32_t n_scenes;
   int64_t capacity;
   // depth-only pass whose output is the depth alone (no primitive ids or
@@ -59,6 +63,19 @@ struct Frame {
   // depth keys kept in the zbuffer itself
   int32_t narrow;
 };
+
+// floor(n / d) for 0 <= n < 2^31 with m = ceil(2^(31 + l) / d), sh = 31 + l,
+// l = ceil(log2 d): exact (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", 1994, theorem 4.2 with N = 31).
+JR_DEV int32_t udiv31(int32_t n, uint32_t m, int32_t sh) {
+  return (int32_t)(((uint64_t)(uint32_t)n * m) >> sh);
+}
+
+// q / h for 0 <= q < 16 and 1 <= h <= 16: (q * ceil(256 / h)) >> 8 is exact,
+// since the excess over q / h is below q / 256 < 1 / h.
+JR_DEV int32_t div_small(int32_t q, int32_t h) {
+  return (q * ((256 + h - 1) / h)) >> 8;
+}
 
 JR_DEV int32_t scene_of_tri(const jr_scene* __restrict__ scenes, int32_t n,
                             int32_t t) {

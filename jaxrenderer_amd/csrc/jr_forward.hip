@@ -157,7 +157,7 @@ JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
 
 JR_DEV Rect tiles_of(const Rect& p, const Frame& f) {
   if (p.tx0 > p.tx1) return p;
-  Rect t = {p.tx0 / f.TW, p.tx1 / f.TW, p.ty0 / f.TH, p.ty1 / f.TH};
+  Rect t = {p.tx0 >> f.tw_sh, p.tx1 >> f.tw_sh, p.ty0 >> f.th_sh, p.ty1 >> f.th_sh};
   return t;
 }
 
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     float4* __restrict__ rec, Rect* __restrict__ rects,
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
-    int32_t* __restrict__ wide, int32_t* __restrict__ wide_count,
+    int2* __restrict__ wide, int32_t* __restrict__ wide_count,
     const float* __restrict__ samples, uint32_t* __restrict__ zkeys,
     uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene) {
   __shared__ SetupStage stage;
@@ -434,7 +434,8 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
         (r.tx0 > r.tx1) ? 0 : (int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
     if (n_t > 0) rects[t] = r;
     if (n_t > kWideTiles) {
-      wide[atomicAdd(wide_count, 1)] = t;  // counted by k_wide (one lane per tile)
+      // counted by k_wide (one lane per tile)
+      wide[atomicAdd(wide_count, 1)] = make_int2(t, s);
       masks[t] = kMaskWide;
     } else {
       // count the tiles that may hold a covered pixel; remember which ones
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
 template <bool BIN>
 __global__ __launch_bounds__(256) void k_wide(
     const jr_scene* __restrict__ scenes, Frame f, const float4* __restrict__ rec,
-    const Rect* __restrict__ rects, const int32_t* __restrict__ wide,
+    const Rect* __restrict__ rects, const int2* __restrict__ wide,
     const int32_t* __restrict__ wide_count, int32_t* __restrict__ counts,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
     int32_t* __restrict__ bins, const float* __restrict__ samples) {
@@ -512,9 +513,9 @@ __global__ __launch_bounds__(256) void k_wide(
   const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int32_t n = *wide_count;
   for (int64_t w = wave; w < n; w += n_waves) {
-    const int32_t t = wide[w];
+    const int2 ts = wide[w];  // (triangle, scene)
+    const int32_t t = ts.x, s = ts.y;
     const Rect r = rects[t];
-    const int32_t s = scene_of_tri_wave(scenes, f.n_scenes, t);
     const float* vp = scenes[s].viewport;
     const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
                  r2 = rec[4 * (int64_t)t + 2];
@@ -625,7 +626,8 @@ __global__ __launch_bounds__(256) void k_bin(
   while (mask) {
     const int q = __ffs(mask) - 1;
     mask &= mask - 1u;
-    const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
+    const int32_t dq = div_small(q, h);
+    const int32_t tx = r.tx0 + dq, ty = r.ty0 + (q - dq * h);
     const int64_t gt = tbase + tx * f.tiles_y + ty;
     const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
     if (pos < f.capacity) bins[pos] = t;
@@ -670,76 +672,85 @@ struct RasterArgs {
 
 // Chooser fallback, shading, merge and stores of pixel (i, j) of scene s
 // whose running minimum over all candidates is (best_key, best_id).
-template <int SHADER>
-JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int32_t i,
-                         int32_t j, float x, float y, uint32_t best_key, int32_t best_id) {
-  const Frame& f = a.f;
-  const float* vp = sc.viewport;
-  const float vp22 = vp[10], vp23 = vp[11];
-  if (best_id == -1) {
-    // narrow form, small winner: its depth is the key's (finite by
-    // construction); the depth shader keeps every fragment
-    const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
-    a.out.zbuffer[pix] = key_depth(best_key) + a.depth_offset;
-    return;
+// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
+// each wavefront of the workgroup works on a tile of its own, a wavefront
+// barrier (a wavefront's LDS operations complete in issue order, so keeping
+// the compiler from moving memory operations across the point is enough).
+template <bool WAVE>
+JR_DEV void tile_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
   }
-  if ((SHADER == JR_SHADER_DEPTH || SHADER == JR_SHADER_VISIBILITY) && !a.out.bary &&
-      best_key < kInfKey && best_key != 0u && best_key != 0x80000000u) {
-    // a depth-only winner whose key is neither NaN nor zero gives its depth
-    // back exactly; no barycentrics are needed, and these shaders keep every
-    // fragment
-    const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
-    const float z = key_depth(best_key);
-    a.out.zbuffer[pix] = a.apply_offset ? z + a.depth_offset : z;
-    if (a.out.prim_id) a.out.prim_id[pix] = best_id;
-    return;
-  }
+}
 
-  // argmin over all primitives returns index 0 of the scene when no candidate
-  // has a finite (or NaN) depth (renderer/shader.py:207-217); the chosen
-  // primitive's own keep & in_triangle then decides (renderer/pipeline.py:275).
+// The primitive a pixel shows: the chooser's winner, or, when no candidate
+// has a finite (or NaN) depth, argmin's index 0 of the scene
+// (renderer/shader.py:207-217), whose own keep & in_triangle then decides
+// (renderer/pipeline.py:275).  With cover set, also the barycentrics, the
+// window depth and the primitive's flags.
+struct PixelHit {
   int32_t w;
   bool cover;
-  if (sc.tri_end <= sc.tri_begin) {
-    w = -1;  // a scene without primitives draws nothing
-    cover = false;
-  } else if (best_key >= kInfKey) {
-    w = sc.tri_begin;
-    const float4 r0 = a.rec[4 * (int64_t)w], r1 = a.rec[4 * (int64_t)w + 1],
-                 r2 = a.rec[4 * (int64_t)w + 2];
-    const uint32_t fl = __float_as_uint(a.rec[4 * (int64_t)w + 3].y);
-    const float e0 = edge(r0, x, y), e1 = edge(r1, x, y), e2 = edge(r2, x, y);
-    cover = (fl & kKeep) && e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f;
-  } else {
-    w = best_id;
-    cover = true;
-  }
+  uint32_t fl;
+  float z;
+  float b[3];
+};
 
-  const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
-  bool keep = false;
-  float z = 0.0f, b[3] = {0.0f, 0.0f, 0.0f}, col[3] = {0.0f, 0.0f, 0.0f};
-  if (cover) {
-    const float4 r0 = a.rec[4 * (int64_t)w], r1 = a.rec[4 * (int64_t)w + 1],
-                 r2 = a.rec[4 * (int64_t)w + 2];
-    const uint32_t fl = __float_as_uint(a.rec[4 * (int64_t)w + 3].y);
+JR_DEV PixelHit resolve_pixel(const RasterArgs& a, const jr_scene& sc, float x, float y,
+                              uint32_t best_key, int32_t best_id) {
+  PixelHit h;
+  h.fl = 0u;
+  h.z = 0.0f;
+  h.b[0] = h.b[1] = h.b[2] = 0.0f;
+  if (sc.tri_end <= sc.tri_begin) {
+    h.w = -1;  // a scene without primitives draws nothing
+    h.cover = false;
+  } else if (best_key >= kInfKey) {
+    h.w = sc.tri_begin;
+    const float4 r0 = a.rec[4 * (int64_t)h.w], r1 = a.rec[4 * (int64_t)h.w + 1],
+                 r2 = a.rec[4 * (int64_t)h.w + 2];
+    const uint32_t fl = __float_as_uint(a.rec[4 * (int64_t)h.w + 3].y);
+    const float e0 = edge(r0, x, y), e1 = edge(r1, x, y), e2 = edge(r2, x, y);
+    h.cover = (fl & kKeep) && e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f;
+  } else {
+    h.w = best_id;
+    h.cover = true;
+  }
+  if (h.cover) {
+    const float4 r0 = a.rec[4 * (int64_t)h.w], r1 = a.rec[4 * (int64_t)h.w + 1],
+                 r2 = a.rec[4 * (int64_t)h.w + 2];
+    h.fl = __float_as_uint(a.rec[4 * (int64_t)h.w + 3].y);
     const float e0 = edge(r0, x, y), e1 = edge(r1, x, y), e2 = edge(r2, x, y);
     float wr = e0 + e1;
     wr = wr + e2;
-    b[0] = e0 / wr;
-    b[1] = e1 / wr;
-    b[2] = e2 / wr;
-    z = win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23);
-    Corner c;
-    c.slot[0] = a.in.faces[3 * (int64_t)w + 0];
-    c.slot[1] = a.in.faces[3 * (int64_t)w + 1];
-    c.slot[2] = a.in.faces[3 * (int64_t)w + 2];
-    keep = Shade<SHADER>::run(a.in, sc, c, b, (fl & kFront) != 0, col, s);
+    h.b[0] = e0 / wr;
+    h.b[1] = e1 / wr;
+    h.b[2] = e2 / wr;
+    h.z = win_depth(r0, r1, r2, e0, e1, e2, sc.viewport[10], sc.viewport[11]);
   }
+  return h;
+}
 
-  // merge (renderer/pipeline.py:401-440) and the shadow-map offset
-  // (renderer/shadow.py:116)
+JR_DEV Corner corner_slots(const jr_inputs& in, int32_t w) {
+  Corner c;
+  c.slot[0] = in.faces[3 * (int64_t)w + 0];
+  c.slot[1] = in.faces[3 * (int64_t)w + 1];
+  c.slot[2] = in.faces[3 * (int64_t)w + 2];
+  return c;
+}
+
+// merge (renderer/pipeline.py:401-440), the shadow-map offset
+// (renderer/shadow.py:116) and the optional id / barycentric / display outputs
+JR_DEV void store_pixel(const RasterArgs& a, int32_t s, int32_t i, int32_t j,
+                        const PixelHit& h, bool keep, const float col[3]) {
+  const Frame& f = a.f;
+  const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
   if (keep) {
-    a.out.zbuffer[pix] = a.apply_offset ? z + a.depth_offset : z;
+    a.out.zbuffer[pix] = a.apply_offset ? h.z + a.depth_offset : h.z;
     if (a.out.canvas) {
       a.out.canvas[3 * pix + 0] = col[0];
       a.out.canvas[3 * pix + 1] = col[1];
@@ -776,41 +787,131 @@ JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int
     d[1] = display_u8(c1);
     d[2] = display_u8(c2);
   }
-  if (a.out.prim_id) a.out.prim_id[pix] = keep ? w : -1;
+  if (a.out.prim_id) a.out.prim_id[pix] = keep ? h.w : -1;
   if (a.out.bary) {
-    a.out.bary[3 * pix + 0] = keep ? b[0] : 0.0f;
-    a.out.bary[3 * pix + 1] = keep ? b[1] : 0.0f;
-    a.out.bary[3 * pix + 2] = keep ? b[2] : 0.0f;
+    a.out.bary[3 * pix + 0] = keep ? h.b[0] : 0.0f;
+    a.out.bary[3 * pix + 1] = keep ? h.b[1] : 0.0f;
+    a.out.bary[3 * pix + 2] = keep ? h.b[2] : 0.0f;
+  }
+}
+
+template <int SHADER>
+JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int32_t i,
+                         int32_t j, float x, float y, uint32_t best_key, int32_t best_id) {
+  const Frame& f = a.f;
+  if (best_id == -1) {
+    // narrow form, small winner: its depth is the key's (finite by
+    // construction); the depth shader keeps every fragment
+    const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
+    a.out.zbuffer[pix] = key_depth(best_key) + a.depth_offset;
+    return;
+  }
+  if ((SHADER == JR_SHADER_DEPTH || SHADER == JR_SHADER_VISIBILITY) && !a.out.bary &&
+      best_key < kInfKey && best_key != 0u && best_key != 0x80000000u) {
+    // a depth-only winner whose key is neither NaN nor zero gives its depth
+    // back exactly; no barycentrics are needed, and these shaders keep every
+    // fragment
+    const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)i * f.H + j;
+    const float z = key_depth(best_key);
+    a.out.zbuffer[pix] = a.apply_offset ? z + a.depth_offset : z;
+    if (a.out.prim_id) a.out.prim_id[pix] = best_id;
+    return;
+  }
+  const PixelHit h = resolve_pixel(a, sc, x, y, best_key, best_id);
+  bool keep = false;
+  float col[3] = {0.0f, 0.0f, 0.0f};
+  if (h.cover)
+    keep = Shade<SHADER>::run(a.in, sc, corner_slots(a.in, h.w), h.b, (h.fl & kFront) != 0,
+                              col, s);
+  store_pixel(a, s, i, j, h, keep, col);
+}
+
+// Open-addressing set of triangle ids in LDS (-1: free): the entry holding w,
+// or -1 when the table is full.
+JR_DEV int32_t table_insert(int32_t* tab, int32_t cap, int32_t w) {
+  int32_t e = (int32_t)((uint32_t)w % (uint32_t)cap);
+  for (int32_t n = 0; n < cap; ++n) {
+    const int32_t cur = atomicCAS(tab + e, -1, w);
+    if (cur == -1 || cur == w) return e;
+    e = (e + 1 == cap) ? 0 : e + 1;
+  }
+  return -1;
+}
+
+// Finish of a tile for a shader with a per-triangle stage: every distinct
+// covered primitive of the tile gets its corners' vertex-stage outputs
+// computed once (by one thread, into LDS: the triangle records' storage,
+// free once visibility is decided), and its pixels read them from there.
+// Pixels of primitives beyond the table's capacity compute their own.
+template <int SHADER, int NT, int PPL, bool WAVE>
+JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t s, int tid,
+                               const int32_t* pi, const int32_t* pj, const float* px,
+                               const float* py, const bool* inside, const uint32_t* best_key,
+                               const int32_t* best_id, float4* lds_rec, int32_t* lds_id) {
+  using V = Varyings<SHADER>;
+  constexpr int NS = V::N | 1;  // odd stride: no LDS bank conflicts
+  constexpr int CAP0 = (48 * NT) / (4 * NS);
+  constexpr int CAP = CAP0 < NT ? CAP0 : NT;
+  float* vary = reinterpret_cast<float*>(lds_rec);
+  int32_t* tab = lds_id;
+  PixelHit h[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    if (inside[q]) {
+      h[q] = resolve_pixel(a, sc, px[q], py[q], best_key[q], best_id[q]);
+    } else {
+      h[q].w = -1;
+      h[q].cover = false;
+    }
+  }
+  tile_sync<WAVE>();  // the visibility loop's reads of the records are done
+  for (int e = tid; e < CAP; e += NT) tab[e] = -1;
+  tile_sync<WAVE>();
+  int32_t slot[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) slot[q] = h[q].cover ? table_insert(tab, CAP, h[q].w) : -1;
+  tile_sync<WAVE>();
+  for (int e = tid; e < CAP; e += NT) {
+    const int32_t w = tab[e];
+    if (w >= 0) {
+      float v[V::N];
+      V::corners(a.in, sc, corner_slots(a.in, w), v);
+#pragma unroll
+      for (int k = 0; k < V::N; ++k) vary[e * NS + k] = v[k];
+    }
+  }
+  tile_sync<WAVE>();
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    if (!inside[q]) continue;
+    bool keep = false;
+    float col[3] = {0.0f, 0.0f, 0.0f};
+    if (h[q].cover) {
+      float v[V::N];
+      if (slot[q] >= 0) {
+#pragma unroll
+        for (int k = 0; k < V::N; ++k) v[k] = vary[slot[q] * NS + k];
+      } else {
+        V::corners(a.in, sc, corner_slots(a.in, h[q].w), v);
+      }
+      keep = V::pixel(a.in, sc, v, h[q].b, (h[q].fl & kFront) != 0, col, s);
+    }
+    store_pixel(a, s, pi[q], pj[q], h[q], keep, col);
   }
 }
 
 // One screen tile of TW x TH pixels with TW * TH / PPL threads: thread tid
 // owns the pixels tid + q * (TW * TH / PPL), q < PPL.  The tile's triangle
 // list is staged through LDS once and tested against all of a thread's pixels.
-// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
-// each wavefront of the workgroup works on a tile of its own, a wavefront
-// barrier (a wavefront's LDS operations complete in issue order, so keeping
-// the compiler from moving memory operations across the point is enough).
-template <bool WAVE>
-JR_DEV void tile_sync() {
-  if (WAVE) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  } else {
-    __syncthreads();
-  }
-}
-
 template <int SHADER, int TW, int TH, int PPL, bool WAVE = false>
 JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
                         int32_t* lds_id) {
   constexpr int NT = TW * TH / PPL;
   static_assert(!WAVE || NT == 64, "a wavefront-owned tile has one lane per 4 pixels");
   const Frame& f = a.f;
-  const int32_t s = (int32_t)(gt / f.T);
-  const int32_t tile = (int32_t)(gt - (int64_t)s * f.T);
-  const int32_t tx = tile / f.tiles_y, ty = tile - tx * f.tiles_y;
+  const int32_t s = udiv31((int32_t)gt, f.T_m, f.T_sh);
+  const int32_t tile = (int32_t)gt - s * f.T;
+  const int32_t tx = udiv31(tile, f.ty_m, f.ty_sh), ty = tile - tx * f.tiles_y;
   const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const jr_scene& sc = a.scenes[s];
   const float* vp = sc.viewport;
@@ -910,10 +1011,15 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
       }
     }
   }
+  if constexpr (Varyings<SHADER>::N > 0) {
+    finish_tile_shared<SHADER, NT, PPL, WAVE>(a, sc, s, tid, pi, pj, px, py, inside, best_key,
+                                              best_id, lds_rec, lds_id);
+  } else {
 #pragma unroll
-  for (int q = 0; q < PPL; ++q)
-    if (inside[q])
-      finish_pixel<SHADER>(a, sc, s, pi[q], pj[q], px[q], py[q], best_key[q], best_id[q]);
+    for (int q = 0; q < PPL; ++q)
+      if (inside[q])
+        finish_pixel<SHADER>(a, sc, s, pi[q], pj[q], px[q], py[q], best_key[q], best_id[q]);
+  }
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by group id, and each
@@ -928,7 +1034,7 @@ constexpr int kXcds = 8;
 JR_DEV int64_t xcd_item(const RasterArgs& a, int32_t x, int64_t k) {
   const int64_t n_x = (int64_t)((a.f.n_scenes - x + kXcds - 1) / kXcds) * a.f.T;
   if (k >= n_x) return -1;
-  const int64_t sl = k / a.f.T;
+  const int64_t sl = udiv31((int32_t)k, a.f.T_m, a.f.T_sh);
   return (sl * kXcds + x) * a.f.T + (k - sl * a.f.T);
 }
 
@@ -1051,6 +1157,20 @@ struct WsLayout {
   int64_t n_tiles, nblk, n_pix;
 };
 
+// m, sh for udiv31 by d (1 <= d < 2^31)
+void magic31(uint32_t d, uint32_t* m, int32_t* sh) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  *m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+  *sh = 31 + l;
+}
+
+int32_t log2_exact(int32_t v) {
+  int32_t l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
 WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   WsLayout L{};
   const int64_t tiles_x = (fr->width + fr->tile_w - 1) / fr->tile_w;
@@ -1067,7 +1187,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.block_sums = o; o += align_up(4 * std::max<int64_t>(L.nblk, 1));
   L.n_pix = (int64_t)fr->width * fr->height * fr->n_scenes;
   L.keys = o;       o += align_up(8 * L.n_pix);
-  L.wide = o;       o += align_up(4 * (int64_t)std::max(n_tris, 1));
+  L.wide = o;       o += align_up(8 * (int64_t)std::max(n_tris, 1));
   L.samples = o;    o += align_up(4 * (int64_t)(fr->width + fr->height) * fr->n_scenes);
   L.masks = o;      o += align_up(4 * (int64_t)std::max(n_tris, 1));
   L.wave_scene = o; o += align_up(4 * (((int64_t)std::max(n_tris, 1) + 63) / 64));
@@ -1087,6 +1207,9 @@ int32_t check_frame(const jr_frame* fr) {
                        (fr->tile_w == 8 && fr->tile_h == 32) ||
                        (fr->tile_w == 4 && fr->tile_h == 16);
   if (!ok_tile) return fail(JR_E_INVALID, "unsupported tile shape");
+  const int64_t n_tiles = (int64_t)((fr->width + fr->tile_w - 1) / fr->tile_w) *
+                          ((fr->height + fr->tile_h - 1) / fr->tile_h) * fr->n_scenes;
+  if (n_tiles > 0x7FFFFFFFLL) return fail(JR_E_INVALID, "more than 2^31 - 1 tiles");
   if (fr->bin_capacity < 0 || fr->bin_capacity > 0x7FFFFFFFLL)
     return fail(JR_E_INVALID, "bin_capacity out of range");
   if (fr->tiles_per_group < 0 || fr->tiles_per_group > 64)
@@ -1247,6 +1370,10 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   f.tiles_x = (f.W + f.TW - 1) / f.TW;
   f.tiles_y = (f.H + f.TH - 1) / f.TH;
   f.T = f.tiles_x * f.tiles_y;
+  f.tw_sh = log2_exact(f.TW);
+  f.th_sh = log2_exact(f.TH);
+  magic31((uint32_t)f.T, &f.T_m, &f.T_sh);
+  magic31((uint32_t)f.tiles_y, &f.ty_m, &f.ty_sh);
   f.n_scenes = fr->n_scenes;
   f.capacity = fr->bin_capacity;
   {
@@ -1266,7 +1393,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   int32_t* block_sums = reinterpret_cast<int32_t*>(ws + L.block_sums);
   int32_t* bins = reinterpret_cast<int32_t*>(ws + L.bins);
   uint64_t* keys = reinterpret_cast<uint64_t*>(ws + L.keys);
-  int32_t* wide = reinterpret_cast<int32_t*>(ws + L.wide);
+  int2* wide = reinterpret_cast<int2*>(ws + L.wide);
   float* samples = reinterpret_cast<float*>(ws + L.samples);
   uint32_t* masks = reinterpret_cast<uint32_t*>(ws + L.masks);
   int32_t* wave_scene = reinterpret_cast<int32_t*>(ws + L.wave_scene);

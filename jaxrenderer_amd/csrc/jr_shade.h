@@ -348,24 +348,40 @@ struct Shade<JR_SHADER_PHONG_DARBOUX> {
   }
 };
 
-// shaders/phong_reflection.py:100-235
+// Shaders whose fragment stage splits into a per-triangle part (the vertex
+// stage outputs of the three corners, packed into N floats) and a per-pixel
+// part.  A tile computes the per-triangle part once for each distinct winning
+// triangle and shares it among that triangle's pixels; the result is the same
+// arithmetic as running both parts for every pixel.  N == 0: not split.
+template <int SHADER>
+struct Varyings {
+  static constexpr int N = 0;
+};
+
+// shaders/phong_reflection.py:100-235: per corner the eye-space normal and
+// uv; the texture index of corner 0.
 template <>
-struct Shade<JR_SHADER_PHONG_REFLECTION> {
-  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc,
-                         const Corner& c, const float b[3], bool front,
-                         float col[3], int) {
-    float ne[3][3], uvv[3][2];
+struct Varyings<JR_SHADER_PHONG_REFLECTION> {
+  static constexpr int N = 16;
+  static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                             float v[N]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], ne[k]);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
+      eye_normal(in, sc, c.slot[k], v + 3 * k);
+      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
     }
-    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+    v[15] = __int_as_float(in.tex_index[attr_row(in.tex_index_index, c.slot[0])]);
+  }
+  static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
+                           const float b[3], bool front, float col[3], int) {
+    const float* ne = v;
+    const float* uvv = v + 9;
+    const int32_t ti = __float_as_int(v[15]);
     float n[3], uv[2];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
-    uv[0] = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    uv[1] = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[q], ne[3 + q], ne[6 + q]);
+    uv[0] = interp(b, uvv[0], uvv[2], uvv[4]);
+    uv[1] = interp(b, uvv[1], uvv[3], uvv[5]);
     float tex[3], diff, spec;
     reflection_terms(in, sc, n, uv, ti, tex, &diff, &spec);
 #pragma unroll
@@ -377,33 +393,40 @@ struct Shade<JR_SHADER_PHONG_REFLECTION> {
   }
 };
 
-// shaders/phong_reflection_shadow.py:110-271
+// shaders/phong_reflection_shadow.py:110-271: as above plus each corner's
+// homogeneous-normalised shadow-camera clip position.
 template <>
-struct Shade<JR_SHADER_PHONG_REFLECTION_SHADOW> {
-  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc,
-                         const Corner& c, const float b[3], bool front,
-                         float col[3], int scene) {
-    float ne[3][3], uvv[3][2], scd[3][4];
+struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
+  static constexpr int N = 28;
+  static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                             float v[N]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], ne[k]);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
+      eye_normal(in, sc, c.slot[k], v + 3 * k);
+      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
       float h[4];
       position_clip(in, sc.shadow_world_to_clip, c.slot[k], h);
       // normalise_homogeneous: every component divided by w
-      scd[k][0] = h[0] / h[3];
-      scd[k][1] = h[1] / h[3];
-      scd[k][2] = h[2] / h[3];
-      scd[k][3] = h[3] / h[3];
+      v[15 + 4 * k + 0] = h[0] / h[3];
+      v[15 + 4 * k + 1] = h[1] / h[3];
+      v[15 + 4 * k + 2] = h[2] / h[3];
+      v[15 + 4 * k + 3] = h[3] / h[3];
     }
-    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+    v[27] = __int_as_float(in.tex_index[attr_row(in.tex_index_index, c.slot[0])]);
+  }
+  static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
+                           const float b[3], bool front, float col[3], int scene) {
+    const float* ne = v;
+    const float* uvv = v + 9;
+    const float* scd = v + 15;
+    const int32_t ti = __float_as_int(v[27]);
     float n[3], uv[2], s4[4];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
-    uv[0] = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    uv[1] = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[q], ne[3 + q], ne[6 + q]);
+    uv[0] = interp(b, uvv[0], uvv[2], uvv[4]);
+    uv[1] = interp(b, uvv[1], uvv[3], uvv[5]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) s4[q] = interp(b, scd[0][q], scd[1][q], scd[2][q]);
+    for (int q = 0; q < 4; ++q) s4[q] = interp(b, scd[q], scd[4 + q], scd[8 + q]);
     // shadow.camera.viewport @ shadow_coord, then normalise_homogeneous
     float sv[4];
     apply4(sc.shadow_viewport, s4[0], s4[1], s4[2], s4[3], sv);
@@ -420,6 +443,28 @@ struct Shade<JR_SHADER_PHONG_REFLECTION_SHADOW> {
       col[ch] = sc.ambient[ch] * tex[ch] + ((shadow * k) * tex[ch]) * sc.light_colour[ch];
     }
     return front;
+  }
+};
+
+template <>
+struct Shade<JR_SHADER_PHONG_REFLECTION> {
+  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                         const float b[3], bool front, float col[3], int scene) {
+    using V = Varyings<JR_SHADER_PHONG_REFLECTION>;
+    float v[V::N];
+    V::corners(in, sc, c, v);
+    return V::pixel(in, sc, v, b, front, col, scene);
+  }
+};
+
+template <>
+struct Shade<JR_SHADER_PHONG_REFLECTION_SHADOW> {
+  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                         const float b[3], bool front, float col[3], int scene) {
+    using V = Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW>;
+    float v[V::N];
+    V::corners(in, sc, c, v);
+    return V::pixel(in, sc, v, b, front, col, scene);
   }
 };
 
