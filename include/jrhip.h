@@ -155,8 +155,19 @@ typedef struct jr_frame {
                               side by side (a wavefront each), else one after another */
   int32_t pixels_per_lane; /* pixels per raster thread: 0/1, or 4 with 16x16 or 8x32
                               tiles (one wavefront per tile) */
-  int32_t pad_i;
+  int32_t part;            /* JR_PART_ALL: every stage; JR_PART_BIN: stages 1-5 (clear,
+                              setup, wide, scan, bin) only; JR_PART_RASTER: stage 6 only,
+                              on a workspace holding this frame's JR_PART_BIN results */
 } jr_frame;
+
+/* jr_frame.part.  Splitting a pass lets a caller overlap one pass's binning
+ * with another pass's raster (the main pass's binning does not read the
+ * shadow map, only its raster does). */
+enum {
+  JR_PART_ALL = 0,
+  JR_PART_BIN = 1,
+  JR_PART_RASTER = 2
+};
 
 /* Gradient buffers for jr_backward (all optional, accumulated with +=). */
 typedef struct jr_grads {

@@ -29,6 +29,8 @@ SHADER_VISIBILITY = 7
 
 # jr_stage_times: JR_STAGES timings per jr_forward call, in this order
 STAGE_NAMES = ("clear", "setup", "wide", "scan", "bin", "raster")
+# jr_frame.part: every stage, the binning stages 1-5 only, the raster only
+PART_ALL, PART_BIN, PART_RASTER = 0, 1, 2
 STAGES = len(STAGE_NAMES)
 
 c_f = ctypes.c_float
@@ -125,7 +127,7 @@ class JrFrame(ctypes.Structure):
         ("persistent", c_i32),
         ("tiles_per_group", c_i32),
         ("pixels_per_lane", c_i32),
-        ("pad_i", c_i32),
+        ("part", c_i32),
     ]
 
 

@@ -164,7 +164,8 @@ def default_capacity(n_tris: int, n_tiles: int) -> int:
 def make_frame(*, width, height, n_scenes, shader, init_buffers=False, apply_offset=False,
                zbuffer_default=1.0, colour_default=(1.0, 1.0, 1.0), depth_offset=0.0,
                tile=DEFAULT_TILE, persistent=False, capacity=0,
-               tiles_per_group: int = 1, pixels_per_lane: int = 1) -> _hip.JrFrame:
+               tiles_per_group: int = 1, pixels_per_lane: int = 1,
+               part: int = _hip.PART_ALL) -> _hip.JrFrame:
     fr = _hip.JrFrame()
     fr.bin_capacity = int(capacity)
     fr.width, fr.height, fr.n_scenes, fr.shader = int(width), int(height), int(n_scenes), int(shader)
@@ -180,6 +181,7 @@ def make_frame(*, width, height, n_scenes, shader, init_buffers=False, apply_off
     fr.persistent = int(bool(persistent))
     fr.tiles_per_group = int(tiles_per_group)
     fr.pixels_per_lane = int(pixels_per_lane)
+    fr.part = int(part)
     return fr
 
 
@@ -215,8 +217,12 @@ def forward(
     capacity: Optional[int] = None,
     stream: Optional[torch.cuda.Stream] = None,
     workspace: Optional[torch.Tensor] = None,
+    part: int = _hip.PART_ALL,
 ) -> None:
-    """One jr_forward call on `stream` (torch's current stream by default)."""
+    """One jr_forward call on `stream` (torch's current stream by default).
+    `part`: every stage, or only the binning stages (PART_BIN) or only the
+    raster (PART_RASTER, on the workspace of a PART_BIN call of the same
+    frame)."""
     L = _hip.lib()
     dev = zbuffer.device
     if dev.type != "cuda":
@@ -228,7 +234,8 @@ def forward(
                     init_buffers=init_buffers, apply_offset=apply_offset,
                     zbuffer_default=zbuffer_default, colour_default=colour_default,
                     depth_offset=depth_offset, tile=tile, persistent=persistent, capacity=cap,
-                    tiles_per_group=tiles_per_group, pixels_per_lane=pixels_per_lane)
+                    tiles_per_group=tiles_per_group, pixels_per_lane=pixels_per_lane,
+                    part=part)
     jin = _hip.JrInputs()
     for name, dt in _INPUT_FIELDS:
         setattr(jin, name, _ptr(inputs.get(name), name, dt, dev))

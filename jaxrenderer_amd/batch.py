@@ -186,7 +186,7 @@ class _Pass:
                                       n_tris=c.n_tris, tile=self.tile, capacity=self.capacity)
         self.workspace = torch.empty(need, dtype=torch.uint8, device=self.zbuffer.device)
 
-    def launch(self) -> None:
+    def launch(self, part: int = _hip.PART_ALL) -> None:
         c = self.call
         _raster.forward(
             shader=c.shader, width=c.width, height=c.height, scenes=self.scenes,
@@ -197,7 +197,7 @@ class _Pass:
             apply_offset=c.apply_offset, depth_offset=c.depth_offset, tile=self.tile,
             persistent=self.persistent, tiles_per_group=self.tiles_per_group,
             pixels_per_lane=self.pixels_per_lane,
-            capacity=self.capacity, workspace=self.workspace)
+            capacity=self.capacity, workspace=self.workspace, part=part)
 
     def pairs(self) -> int:
         c = self.call
@@ -224,6 +224,7 @@ class BatchRenderer:
         if main.call.shadow_from is not None:
             main.inputs["shadow_maps"] = self.passes[main.call.shadow_from].zbuffer
         self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self._side: Optional[torch.cuda.Stream] = None
 
     def render(self) -> Dict[str, torch.Tensor]:
         for p in self.passes.values():
@@ -258,13 +259,35 @@ class BatchRenderer:
 
         return autotune.tune(self, **kw)
 
+    def render_overlapped(self) -> None:
+        """render() with the main pass's binning on a second stream, beside
+        the shadow pass (it reads no shadow map); only the main raster waits
+        for the shadow map.  Same results as render()."""
+        if "shadow" not in self.passes or len(self.passes) != 2:
+            self.render()
+            return
+        shadow, main = self.passes["shadow"], self.passes["main"]
+        cur = torch.cuda.current_stream()
+        if self._side is None:
+            self._side = torch.cuda.Stream()  # default priority
+        side = self._side
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            main.launch(_hip.PART_BIN)
+        shadow.launch()
+        side.wait_stream(cur)  # the shadow map is complete
+        with torch.cuda.stream(side):
+            main.launch(_hip.PART_RASTER)
+        cur.wait_stream(side)
+
     def capture(self) -> None:
-        """Captures render() into a HIP graph (torch.cuda.CUDAGraph)."""
-        self.render()
+        """Captures render_overlapped() into a HIP graph
+        (torch.cuda.CUDAGraph)."""
+        self.render_overlapped()
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self.render()
+            self.render_overlapped()
         self.graph = g
 
     def replay(self) -> Dict[str, torch.Tensor]:

@@ -1125,9 +1125,10 @@ struct StageTimer {
   hipStream_t st;
   std::array<hipEvent_t, JR_STAGES + 1> ev{};
   bool on = false;
-  explicit StageTimer(hipStream_t s) : st(s) {
+  // only calls that run every stage are timed
+  StageTimer(hipStream_t s, bool whole) : st(s) {
     std::lock_guard<std::mutex> g(g_timing_mu);
-    on = g_timing;
+    on = g_timing && whole;
     if (!on) return;
     for (auto& e : ev)
       if (hipEventCreate(&e) != hipSuccess) e = nullptr;
@@ -1212,6 +1213,8 @@ int32_t check_frame(const jr_frame* fr) {
   if (n_tiles > 0x7FFFFFFFLL) return fail(JR_E_INVALID, "more than 2^31 - 1 tiles");
   if (fr->bin_capacity < 0 || fr->bin_capacity > 0x7FFFFFFFLL)
     return fail(JR_E_INVALID, "bin_capacity out of range");
+  if (fr->part != JR_PART_ALL && fr->part != JR_PART_BIN && fr->part != JR_PART_RASTER)
+    return fail(JR_E_INVALID, "part must be JR_PART_ALL, JR_PART_BIN or JR_PART_RASTER");
   if (fr->tiles_per_group < 0 || fr->tiles_per_group > 64)
     return fail(JR_E_INVALID, "tiles_per_group must be in [0, 64]");
   const bool ok_ppl = fr->pixels_per_lane <= 1 ||
@@ -1400,7 +1403,9 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   int32_t* wide_count = misc + 2;
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
-  StageTimer timer(st);
+  StageTimer timer(st, fr->part == JR_PART_ALL);
+  const int n_tris = in->n_tris;
+  if (fr->part != JR_PART_RASTER) {
   // counts, cursor and misc are contiguous: clear them, and the key buffer,
   // with one fill kernel (a kernel rather than hipMemsetAsync so the sequence
   // can be graph-captured)
@@ -1413,7 +1418,6 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
                      zkeys, f.narrow ? L.n_pix : (int64_t)0, scenes, f, samples, wave_scene,
                      in->n_tris);
   timer.mark(1);
-  const int n_tris = in->n_tris;
   if (n_tris > 0)
     hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
                        dim3(kSetupThreads), 0, st, *in,
@@ -1438,6 +1442,13 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
                        rects, wide, wide_count, counts, offsets, cursor, bins, samples);
   }
   timer.mark(5);
+  }
+  if (fr->part == JR_PART_BIN) {
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+      return fail(JR_E_HIP, std::string("launch failed: ") + hipGetErrorString(err));
+    return JR_OK;
+  }
   RasterArgs a;
   a.in = *in;
   a.scenes = scenes;
