@@ -233,28 +233,30 @@ struct ShadeBwd<JR_SHADER_PHONG> {
 // shaders/phong_reflection.py:154-235 and phong_reflection_shadow.py:173-271
 template <bool SHADOW>
 struct ReflectionBwd {
+  using V = Varyings<SHADOW ? JR_SHADER_PHONG_REFLECTION_SHADOW : JR_SHADER_PHONG_REFLECTION>;
   static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
                          const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
                          int scene) {
-    float ne[3][3], uvv[3][2], scd[3][4];
-    for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], ne[k]);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
-      if (SHADOW) {
-        float h[4];
-        position_clip(in, sc.shadow_world_to_clip, c.slot[k], h);
-        for (int q = 0; q < 4; ++q) scd[k][q] = h[q] / h[3];
-      }
-    }
-    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+    float v[V::N];
+    V::corners(in, sc, c, v);
+    run_v(in, sc, gr, v, b, dcol, g, scene);
+  }
+  // v: the triangle's corner stage (Varyings<>::corners)
+  static JR_DEV void run_v(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
+                           const float* vary, const float b[3], const float dcol[3],
+                           PixGrad& g, int scene) {
+    const float* nev = vary;      // eye-space normal of corner k: nev[3k..3k+2]
+    const float* uvv = vary + 9;  // uv of corner k: uvv[2k..2k+1]
+    const int32_t ti = __float_as_int(vary[V::N - 1]);
     float n[3], uv[2];
-    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
-    uv[0] = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    uv[1] = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, nev[q], nev[3 + q], nev[6 + q]);
+    uv[0] = interp(b, uvv[0], uvv[2], uvv[4]);
+    uv[1] = interp(b, uvv[1], uvv[3], uvv[5]);
     bool lit = true;
     if (SHADOW) {
+      const float* scd = vary + 15;  // homogeneous-normalised shadow clip of corner k
       float s4[4], sv[4];
-      for (int q = 0; q < 4; ++q) s4[q] = interp(b, scd[0][q], scd[1][q], scd[2][q]);
+      for (int q = 0; q < 4; ++q) s4[q] = interp(b, scd[q], scd[4 + q], scd[8 + q]);
       apply4(sc.shadow_viewport, s4[0], s4[1], s4[2], s4[3], sv);
       const float* map = in.shadow_maps + (int64_t)scene * in.shadow_w * in.shadow_h;
       lit = (sv[2] / sv[3]) <= shadow_get(map, in.shadow_w, in.shadow_h, sv[0] / sv[3],
@@ -335,7 +337,7 @@ struct ReflectionBwd {
     normalise_bwd(sc.light_dir_eye, dlh, dle);
     for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR_EYE + q] += dle[q];
     for (int k = 0; k < 3; ++k) {
-      g.db[k] = dot3(dn, ne[k]);
+      g.db[k] = dot3(dn, nev + 3 * k);
       for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dn[q];
     }
   }
@@ -352,33 +354,33 @@ JR_DEV float wave_sum(float v) {
   return v;
 }
 
-// d[at + c] += v[c] (c < n) for every lane with at >= 0; called with all 64
-// lanes of the wavefront active.  Lanes that hit the same entry are summed
-// first (a wavefront reduction per distinct entry, for up to 8 entries), so a
-// magnified texture -- many pixels on one texel -- costs one global atomic per
-// texel and tile instead of one per pixel; the remaining lanes add directly.
-JR_DEV void scatter_add(float* __restrict__ d, int64_t at, int32_t n, const float* v) {
-  if (!d) return;
-  const int lane = threadIdx.x & 63;
-  bool want = at >= 0;
-  uint64_t pending = __ballot(want);
-  for (int it = 0; pending && it < 8; ++it) {
-    const int l = __ffsll((unsigned long long)pending) - 1;
-    const int32_t lo = __shfl((int32_t)(uint32_t)(uint64_t)at, l);
-    const int32_t hi = __shfl((int32_t)(uint32_t)((uint64_t)at >> 32), l);
-    const int64_t k = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-    const int32_t nk = __shfl(n, l);
-    const bool mine = want && at == k;
-    pending &= ~__ballot(mine);
-    for (int c = 0; c < 3; ++c) {
-      if (c >= nk) break;
-      const float sum = wave_sum(mine ? v[c] : 0.0f);
-      if (lane == l) atomicAdd(d + k + c, sum);
-    }
-    if (mine) want = false;
+// d[at + c] += v[c] (c < n), summed first in an LDS table of (index, sum)
+// entries shared by the tile's pixels, so a texel that several pixels read
+// costs one global atomic per channel and tile; indices beyond the table's
+// capacity (or beyond 2^31) are added directly.
+template <int CAP, int NC>
+JR_DEV void stage_add(int32_t* tab, float (*val)[NC], float* __restrict__ d, int64_t at,
+                      int32_t n, const float* v) {
+  if (!d || at < 0) return;
+  const int32_t e = at < 0x7FFFFFFFLL ? table_insert(tab, CAP, (int32_t)at) : -1;
+  for (int c = 0; c < n; ++c) {
+    if (v[c] == 0.0f) continue;
+    if (e >= 0)
+      atomicAdd(&val[e][c], v[c]);
+    else
+      atomicAdd(d + at + c, v[c]);
   }
-  if (want)
-    for (int c = 0; c < n; ++c) atomicAdd(d + at + c, v[c]);
+}
+
+// Adds the staged sums to d (n channels per entry) and leaves the table free.
+template <int CAP, int NC>
+JR_DEV void flush_adds(int32_t* tab, float (*val)[NC], float* __restrict__ d, int32_t n) {
+  for (int e = threadIdx.x; e < CAP; e += blockDim.x) {
+    const int32_t at = tab[e];
+    if (at >= 0 && d)
+      for (int c = 0; c < n; ++c)
+        if (val[e][c] != 0.0f) atomicAdd(d + at + c, val[e][c]);
+  }
 }
 
 struct BwdArgs {
@@ -392,6 +394,8 @@ struct BwdArgs {
   int32_t* touched;         // per triangle: 1 once a pixel adds to it, zero between calls
   uint32_t* header;         // workspace header (kHdr* words)
   int32_t W, H, tiles_x, tiles_y, T, n_scenes;
+  uint32_t T_m, ty_m;  // udiv31 by T and tiles_y
+  int32_t T_sh, ty_sh;
 };
 
 // Workspace header words.  The accumulators and flags are left zero by every
@@ -400,113 +404,195 @@ struct BwdArgs {
 constexpr int kHdrMagicLo = 0, kHdrMagicHi = 1, kHdrTris = 2, kHdrFill = 3;
 constexpr uint32_t kMagicLo = 0x6a724277u, kMagicHi = 0x73706b31u;
 
+// One wavefront per 16x16 tile, lane l owning pixels (16 tx + l / 16 + 4 q,
+// 16 ty + l % 16), q < 4.  The tile's distinct primitives (up to kBwdCap)
+// get their setup (M^-1 columns and clip z) and, for the split shaders, their
+// corner stage computed once, by one lane each, into LDS; their per-triangle
+// cotangents are summed in LDS and added to the global accumulators once per
+// (tile, triangle).  Pixels of primitives beyond the table's capacity compute
+// their own and add directly.
+constexpr int kBwdTile = 16;
+constexpr int kBwdCap = 32;
+constexpr int kTexCap = 128;  // distinct texels summed in LDS per tile
+constexpr int kSpecCap = 64;  // distinct specular-map entries
+
 template <int SHADER>
 __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
-  __shared__ float acc[64][kAcc + 1];
+  constexpr int NV = Varyings<SHADER>::N;
+  constexpr int NS = (12 + NV) | 1;  // record stride (odd: no LDS bank conflicts)
+  __shared__ float trec[kBwdCap * NS];
+  __shared__ float acc[kBwdCap][kAcc + 1];
+  __shared__ int32_t tab[kBwdCap];
+  __shared__ int32_t tex_tab[kTexCap];
+  __shared__ float tex_val[kTexCap][3];
+  __shared__ int32_t spec_tab[kSpecCap];
+  __shared__ float spec_val[kSpecCap][1];
   const int lane = threadIdx.x;
-  const int64_t gt = blockIdx.x;
-  const int32_t s = (int32_t)(gt / a.T);
-  const int32_t tile = (int32_t)(gt - (int64_t)s * a.T);
-  const int32_t tx = tile / a.tiles_y, ty = tile - tx * a.tiles_y;
-  const int32_t i = tx * 8 + lane / 8, j = ty * 8 + lane % 8;
+  const int32_t gt = (int32_t)blockIdx.x;
+  const int32_t s = udiv31(gt, a.T_m, a.T_sh);
+  const int32_t tile = gt - s * a.T;
+  const int32_t tx = udiv31(tile, a.ty_m, a.ty_sh), ty = tile - tx * a.tiles_y;
+  const int32_t j = ty * kBwdTile + lane % kBwdTile;
   const jr_scene& sc = a.scenes[s];
   const float* vp = sc.viewport;
-  const int64_t pix = (int64_t)s * a.W * a.H + (int64_t)i * a.H + j;
-  int32_t t = -1;
-  if (i < a.W && j < a.H) t = a.prim_id[pix];
+  auto pixel_of = [&](int q, int32_t* i_out) -> int32_t {
+    const int32_t i = tx * kBwdTile + lane / kBwdTile + 4 * q;
+    *i_out = i;
+    if (i >= a.W || j >= a.H) return -1;
+    return a.prim_id[(int64_t)s * a.W * a.H + (int64_t)i * a.H + j];
+  };
+
+  // the tile's distinct primitives; the texel and specular-map sum tables
+  if (lane < kBwdCap) tab[lane] = -1;
+  for (int e = lane; e < kTexCap; e += 64) {
+    tex_tab[e] = -1;
+    tex_val[e][0] = tex_val[e][1] = tex_val[e][2] = 0.0f;
+  }
+  for (int e = lane; e < kSpecCap; e += 64) {
+    spec_tab[e] = -1;
+    spec_val[e][0] = 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int32_t i;
+    const int32_t t = pixel_of(q, &i);
+    if (t >= 0) (void)table_insert(tab, kBwdCap, t);
+  }
+  __syncthreads();
+  if (lane < kBwdCap) {
+    for (int k = 0; k < kAcc; ++k) acc[lane][k] = 0.0f;
+    const int32_t t = tab[lane];
+    if (t >= 0) {
+      TriSetup ts;
+      tri_setup(a.in, sc, t, ts);
+      float* r = trec + lane * NS;
+      for (int k = 0; k < 3; ++k) {
+        const float4 c = rec_col(ts, k);
+        r[4 * k + 0] = c.x;
+        r[4 * k + 1] = c.y;
+        r[4 * k + 2] = c.z;
+        r[4 * k + 3] = c.w;
+      }
+      if constexpr (NV > 0) {
+        float v[NV > 0 ? NV : 1];
+        Varyings<SHADER>::corners(a.in, sc, corner_slots(a.in, t), v);
+        for (int k = 0; k < NV; ++k) r[12 + k] = v[k];
+      }
+    }
+  }
+  __syncthreads();
 
   PixGrad g;
   for (int q = 0; q < S_COUNT; ++q) g.sg[q] = 0.0f;
-  g.tex_at = -1;
-  g.spec_at = -1;
-  g.tex_n = 0;
-  g.spec_g = 0.0f;
-  for (int q = 0; q < 3; ++q) g.tex_g[q] = 0.0f;
-  float vals[kAcc];
-  for (int q = 0; q < kAcc; ++q) vals[q] = 0.0f;
-  if (t >= 0) {
-    TriSetup ts;
-    tri_setup(a.in, sc, t, ts);
-    const float4 r0 = rec_col(ts, 0), r1 = rec_col(ts, 1), r2 = rec_col(ts, 2);
-    const float x = pix_x(i, vp), y = pix_y(j, vp);
-    const float e[3] = {edge(r0, x, y), edge(r1, x, y), edge(r2, x, y)};
-    float wr = e[0] + e[1];
-    wr = wr + e[2];
-    const float b[3] = {e[0] / wr, e[1] / wr, e[2] / wr};
-    for (int k = 0; k < 3; ++k) {
-      g.db[k] = 0.0f;
-      for (int q = 0; q < 3; ++q) g.dvar[k][q] = 0.0f;
-    }
-    Corner c;
-    c.slot[0] = a.in.faces[3 * (int64_t)t + 0];
-    c.slot[1] = a.in.faces[3 * (int64_t)t + 1];
-    c.slot[2] = a.in.faces[3 * (int64_t)t + 2];
-    if (a.d_canvas) {
-      const float dcol[3] = {a.d_canvas[3 * pix], a.d_canvas[3 * pix + 1],
-                             a.d_canvas[3 * pix + 2]};
-      ShadeBwd<SHADER>::run(a.in, sc, a.gr, c, b, dcol, g, s);
-    }
-    // b = e / sum(e)
-    const float tdb = (g.db[0] * e[0] + g.db[1] * e[1]) + g.db[2] * e[2];
-    float de[3];
-    for (int k = 0; k < 3; ++k) de[k] = g.db[k] / wr - tdb / (wr * wr);
-    // zbuffer = (sum_k e_k z_k) * vp22 + vp23
-    if (a.d_zbuffer) {
-      const float dz = a.d_zbuffer[pix];
-      float zc = e[0] * r0.w;
-      zc = zc + e[1] * r1.w;
-      zc = zc + e[2] * r2.w;
-      const float zk[3] = {r0.w, r1.w, r2.w};
-      for (int k = 0; k < 3; ++k) {
-        de[k] += dz * vp[10] * zk[k];
-        vals[A_Z + k] = dz * vp[10] * e[k];
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+    int32_t i;
+    const int32_t t = pixel_of(q, &i);
+    const int32_t e_slot = t >= 0 ? table_insert(tab, kBwdCap, t) : -1;  // finds its entry
+    const int64_t pix = (int64_t)s * a.W * a.H + (int64_t)i * a.H + j;
+    g.tex_at = -1;
+    g.spec_at = -1;
+    g.tex_n = 0;
+    g.spec_g = 0.0f;
+    for (int k = 0; k < 3; ++k) g.tex_g[k] = 0.0f;
+    float vals[kAcc];
+    for (int k = 0; k < kAcc; ++k) vals[k] = 0.0f;
+    if (t >= 0) {
+      float4 r0, r1, r2;
+      float v[NV > 0 ? NV : 1];
+      if (e_slot >= 0) {
+        const float* r = trec + e_slot * NS;
+        r0 = make_float4(r[0], r[1], r[2], r[3]);
+        r1 = make_float4(r[4], r[5], r[6], r[7]);
+        r2 = make_float4(r[8], r[9], r[10], r[11]);
+        if constexpr (NV > 0)
+          for (int k = 0; k < NV; ++k) v[k] = r[12 + k];
+      } else {
+        TriSetup ts;
+        tri_setup(a.in, sc, t, ts);
+        r0 = rec_col(ts, 0);
+        r1 = rec_col(ts, 1);
+        r2 = rec_col(ts, 2);
+        if constexpr (NV > 0) Varyings<SHADER>::corners(a.in, sc, corner_slots(a.in, t), v);
       }
-      g.sg[S_VP22] += dz * zc;
-      g.sg[S_VP23] += dz;
+      const float x = pix_x(i, vp), y = pix_y(j, vp);
+      const float e[3] = {edge(r0, x, y), edge(r1, x, y), edge(r2, x, y)};
+      float wr = e[0] + e[1];
+      wr = wr + e[2];
+      const float b[3] = {e[0] / wr, e[1] / wr, e[2] / wr};
+      for (int k = 0; k < 3; ++k) {
+        g.db[k] = 0.0f;
+        for (int c = 0; c < 3; ++c) g.dvar[k][c] = 0.0f;
+      }
+      if (a.d_canvas) {
+        const float dcol[3] = {a.d_canvas[3 * pix], a.d_canvas[3 * pix + 1],
+                               a.d_canvas[3 * pix + 2]};
+        if constexpr (NV > 0)
+          ReflectionBwd<SHADER == JR_SHADER_PHONG_REFLECTION_SHADOW>::run_v(
+              a.in, sc, a.gr, v, b, dcol, g, s);
+        else
+          ShadeBwd<SHADER>::run(a.in, sc, a.gr, corner_slots(a.in, t), b, dcol, g, s);
+      }
+      // b = e / sum(e)
+      const float tdb = (g.db[0] * e[0] + g.db[1] * e[1]) + g.db[2] * e[2];
+      float de[3];
+      for (int k = 0; k < 3; ++k) de[k] = g.db[k] / wr - tdb / (wr * wr);
+      // zbuffer = (sum_k e_k z_k) * vp22 + vp23
+      if (a.d_zbuffer) {
+        const float dz = a.d_zbuffer[pix];
+        float zc = e[0] * r0.w;
+        zc = zc + e[1] * r1.w;
+        zc = zc + e[2] * r2.w;
+        const float zk[3] = {r0.w, r1.w, r2.w};
+        for (int k = 0; k < 3; ++k) {
+          de[k] += dz * vp[10] * zk[k];
+          vals[A_Z + k] = dz * vp[10] * e[k];
+        }
+        g.sg[S_VP22] += dz * zc;
+        g.sg[S_VP23] += dz;
+      }
+      // e_k = x Minv[0][k] + y Minv[1][k] + Minv[2][k]
+      float dx = 0.0f, dy = 0.0f;
+      const float m0[3] = {r0.x, r1.x, r2.x}, m1[3] = {r0.y, r1.y, r2.y};
+      for (int k = 0; k < 3; ++k) {
+        vals[A_MINV + 0 * 3 + k] = x * de[k];
+        vals[A_MINV + 1 * 3 + k] = y * de[k];
+        vals[A_MINV + 2 * 3 + k] = de[k];
+        dx += m0[k] * de[k];
+        dy += m1[k] * de[k];
+      }
+      // x = (i - vp03) / vp00, y = (j - vp13) / vp11
+      g.sg[S_VP03] += -dx / vp[0];
+      g.sg[S_VP00] += -dx * x / vp[0];
+      g.sg[S_VP13] += -dy / vp[5];
+      g.sg[S_VP11] += -dy * y / vp[5];
+      for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c) vals[A_VAR + 3 * k + c] = g.dvar[k][c];
+      if (e_slot >= 0) {
+        for (int k = 0; k < kAcc; ++k)
+          if (vals[k] != 0.0f) atomicAdd(&acc[e_slot][k], vals[k]);
+      } else {
+        a.touched[t] = 1;  // k_bwd_tris only visits touched triangles
+        for (int k = 0; k < kAcc; ++k)
+          if (vals[k] != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + k, vals[k]);
+      }
     }
-    // e_k = x Minv[0][k] + y Minv[1][k] + Minv[2][k]
-    float dx = 0.0f, dy = 0.0f;
-    const float m0[3] = {r0.x, r1.x, r2.x}, m1[3] = {r0.y, r1.y, r2.y};
-    for (int k = 0; k < 3; ++k) {
-      vals[A_MINV + 0 * 3 + k] = x * de[k];
-      vals[A_MINV + 1 * 3 + k] = y * de[k];
-      vals[A_MINV + 2 * 3 + k] = de[k];
-      dx += m0[k] * de[k];
-      dy += m1[k] * de[k];
-    }
-    // x = (i - vp03) / vp00, y = (j - vp13) / vp11
-    g.sg[S_VP03] += -dx / vp[0];
-    g.sg[S_VP00] += -dx * x / vp[0];
-    g.sg[S_VP13] += -dy / vp[5];
-    g.sg[S_VP11] += -dy * y / vp[5];
-    for (int k = 0; k < 3; ++k)
-      for (int q = 0; q < 3; ++q) vals[A_VAR + 3 * k + q] = g.dvar[k][q];
+    stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, g.tex_at, g.tex_n, g.tex_g);
+    stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, g.spec_at, 1, &g.spec_g);
   }
-
-  scatter_add(a.gr.d_textures, g.tex_at, g.tex_n, g.tex_g);
-  scatter_add(a.gr.d_specular_maps, g.spec_at, 1, &g.spec_g);
-
-  // group the lanes by triangle: leader = lowest lane with the same id
-  int leader = lane;
-  uint64_t pending = __ballot(t >= 0);
-  while (pending) {
-    const int l = __ffsll((unsigned long long)pending) - 1;
-    const int32_t id = __shfl(t, l);
-    const uint64_t same = __ballot(t == id);
-    if (t == id) leader = l;
-    pending &= ~same;
-  }
-  for (int q = 0; q < kAcc; ++q) acc[lane][q] = 0.0f;
   __syncthreads();
-  if (t >= 0)
-    for (int q = 0; q < kAcc; ++q)
-      if (vals[q] != 0.0f) atomicAdd(&acc[leader][q], vals[q]);
-  __syncthreads();
-  if (t >= 0 && leader == lane) {
-    a.touched[t] = 1;  // k_bwd_tris only visits touched triangles
-    for (int q = 0; q < kAcc; ++q) {
-      const float v = acc[lane][q];
-      if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + q, v);
+  // one texel's channels: tex_c is the scene's (one scene per tile)
+  flush_adds<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, sc.tex_c > 1 ? 3 : 1);
+  flush_adds<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, 1);
+  if (lane < kBwdCap) {
+    const int32_t t = tab[lane];
+    if (t >= 0) {
+      a.touched[t] = 1;
+      for (int k = 0; k < kAcc; ++k) {
+        const float v = acc[lane][k];
+        if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + k, v);
+      }
     }
   }
   // per-scene parameters: one reduction per tile
@@ -714,6 +800,7 @@ __global__ void k_bwd_fill(const uint32_t* __restrict__ hdr, uint32_t* __restric
     p[i] = 0u;
 }
 
+
 thread_local std::string g_bwd_err;
 
 int32_t bfail(int32_t code, const char* msg) {
@@ -800,10 +887,13 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
   a.tacc = reinterpret_cast<float*>(ws + WL.tacc);
   a.W = fr->width;
   a.H = fr->height;
-  a.tiles_x = (a.W + 7) / 8;
-  a.tiles_y = (a.H + 7) / 8;
+  a.tiles_x = (a.W + kBwdTile - 1) / kBwdTile;
+  a.tiles_y = (a.H + kBwdTile - 1) / kBwdTile;
   a.T = a.tiles_x * a.tiles_y;
   a.n_scenes = fr->n_scenes;
+  if ((int64_t)a.T * a.n_scenes > 0x7FFFFFFFLL) return bfail(JR_E_INVALID, "too many tiles");
+  magic31((uint32_t)a.T, &a.T_m, &a.T_sh);
+  magic31((uint32_t)a.tiles_y, &a.ty_m, &a.ty_sh);
   if (in->n_tris == 0) return JR_OK;
   // flags and accumulators: everything after the header
   const int64_t nwords = (WL.total - WL.touched) / 4;

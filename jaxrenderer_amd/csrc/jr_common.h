@@ -71,6 +71,14 @@ JR_DEV int32_t udiv31(int32_t n, uint32_t m, int32_t sh) {
   return (int32_t)(((uint64_t)(uint32_t)n * m) >> sh);
 }
 
+// m, sh of udiv31 for the divisor d (1 <= d < 2^31), on the host.
+inline void magic31(uint32_t d, uint32_t* m, int32_t* sh) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  *m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+  *sh = 31 + l;
+}
+
 // q / h for 0 <= q < 16 and 1 <= h <= 16: (q * ceil(256 / h)) >> 8 is exact,
 // since the excess over q / h is below q / 256 < 1 / h.
 JR_DEV int32_t div_small(int32_t q, int32_t h) {
@@ -121,6 +129,27 @@ JR_DEV int32_t scene_of_wave(const jr_scene* __restrict__ scenes, int32_t n,
     if (t >= nb) s = su;
   }
   return s;
+}
+
+// Open-addressing set of triangle ids in LDS (-1: free): the entry holding w,
+// or -1 when the table is full.
+JR_DEV int32_t table_insert(int32_t* tab, int32_t cap, int32_t w) {
+  int32_t e = (int32_t)((uint32_t)w % (uint32_t)cap);
+  for (int32_t n = 0; n < cap; ++n) {
+    const int32_t cur = atomicCAS(tab + e, -1, w);
+    if (cur == -1 || cur == w) return e;
+    e = (e + 1 == cap) ? 0 : e + 1;
+  }
+  return -1;
+}
+
+// The slots of face row w's corners.
+JR_DEV Corner corner_slots(const jr_inputs& in, int32_t w) {
+  Corner c;
+  c.slot[0] = in.faces[3 * (int64_t)w + 0];
+  c.slot[1] = in.faces[3 * (int64_t)w + 1];
+  c.slot[2] = in.faces[3 * (int64_t)w + 2];
+  return c;
 }
 
 // PerPrimitive.create (renderer/pipeline.py:76-113) for face row t: clip

@@ -735,13 +735,6 @@ JR_DEV PixelHit resolve_pixel(const RasterArgs& a, const jr_scene& sc, float x, 
   return h;
 }
 
-JR_DEV Corner corner_slots(const jr_inputs& in, int32_t w) {
-  Corner c;
-  c.slot[0] = in.faces[3 * (int64_t)w + 0];
-  c.slot[1] = in.faces[3 * (int64_t)w + 1];
-  c.slot[2] = in.faces[3 * (int64_t)w + 2];
-  return c;
-}
 
 // merge (renderer/pipeline.py:401-440), the shadow-map offset
 // (renderer/shadow.py:116) and the optional id / barycentric / display outputs
@@ -826,17 +819,6 @@ JR_DEV void finish_pixel(const RasterArgs& a, const jr_scene& sc, int32_t s, int
   store_pixel(a, s, i, j, h, keep, col);
 }
 
-// Open-addressing set of triangle ids in LDS (-1: free): the entry holding w,
-// or -1 when the table is full.
-JR_DEV int32_t table_insert(int32_t* tab, int32_t cap, int32_t w) {
-  int32_t e = (int32_t)((uint32_t)w % (uint32_t)cap);
-  for (int32_t n = 0; n < cap; ++n) {
-    const int32_t cur = atomicCAS(tab + e, -1, w);
-    if (cur == -1 || cur == w) return e;
-    e = (e + 1 == cap) ? 0 : e + 1;
-  }
-  return -1;
-}
 
 // Finish of a tile for a shader with a per-triangle stage: every distinct
 // covered primitive of the tile gets its corners' vertex-stage outputs
@@ -1158,13 +1140,6 @@ struct WsLayout {
   int64_t n_tiles, nblk, n_pix;
 };
 
-// m, sh for udiv31 by d (1 <= d < 2^31)
-void magic31(uint32_t d, uint32_t* m, int32_t* sh) {
-  int l = 0;
-  while ((1ull << l) < d) ++l;
-  *m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
-  *sh = 31 + l;
-}
 
 int32_t log2_exact(int32_t v) {
   int32_t l = 0;
