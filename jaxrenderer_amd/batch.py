@@ -229,15 +229,7 @@ class BatchRenderer:
     def render(self) -> Dict[str, torch.Tensor]:
         for p in self.passes.values():
             p.launch()
-        main = self.passes["main"]
-        out = {"zbuffer": main.zbuffer, "canvas": main.canvas}
-        if main.prim_id is not None:
-            out.update(prim_id=main.prim_id, bary=main.bary)
-        if main.display is not None:
-            out["display"] = main.display
-        if "shadow" in self.passes:
-            out["shadow_map"] = self.passes["shadow"].zbuffer
-        return out
+        return self._outputs()
 
     def calibrate(self, slack: float = 1.25) -> Dict[str, int]:
         """Runs once and sizes every pass's bin capacity to the binned pairs
@@ -259,13 +251,23 @@ class BatchRenderer:
 
         return autotune.tune(self, **kw)
 
-    def render_overlapped(self) -> None:
+    def _outputs(self) -> Dict[str, torch.Tensor]:
+        main = self.passes["main"]
+        out = {"zbuffer": main.zbuffer, "canvas": main.canvas}
+        if main.prim_id is not None:
+            out.update(prim_id=main.prim_id, bary=main.bary)
+        if main.display is not None:
+            out["display"] = main.display
+        if "shadow" in self.passes:
+            out["shadow_map"] = self.passes["shadow"].zbuffer
+        return out
+
+    def render_overlapped(self) -> Dict[str, torch.Tensor]:
         """render() with the main pass's binning on a second stream, beside
         the shadow pass (it reads no shadow map); only the main raster waits
         for the shadow map.  Same results as render()."""
         if "shadow" not in self.passes or len(self.passes) != 2:
-            self.render()
-            return
+            return self.render()
         shadow, main = self.passes["shadow"], self.passes["main"]
         cur = torch.cuda.current_stream()
         if self._side is None:
@@ -279,6 +281,7 @@ class BatchRenderer:
         with torch.cuda.stream(side):
             main.launch(_hip.PART_RASTER)
         cur.wait_stream(side)
+        return self._outputs()
 
     def capture(self) -> None:
         """Captures render_overlapped() into a HIP graph
@@ -348,7 +351,7 @@ class BatchRenderer:
     def train_step(self, target: torch.Tensor) -> Dict[str, torch.Tensor]:
         """Forward, the gradient of the L2 image loss sum((canvas - target)^2),
         and backward."""
-        out = self.render()
+        out = self.render_overlapped()
         self._d_canvas = 2.0 * (out["canvas"] - target)
         return self.backward(self._d_canvas)
 

@@ -19,15 +19,16 @@
 //   k_bwd_prepare / k_bwd_fill  check the workspace header; clear the
 //                 per-triangle flags and accumulators only for a workspace that
 //                 no earlier call left zero (each call clears what it used)
-//   k_bwd_pixels  one 64-lane group per 8x8 tile: per pixel, shading backward
-//                 (texture / specular scatter, per-scene reductions) and the
-//                 visibility backward (dL/dM^-1, dL/dz_clip, per-corner varying
-//                 cotangents), grouped per triangle in LDS before one set of
-//                 global atomics per (tile, triangle)
-//   k_bwd_tris    one thread per triangle, work for the touched ones: M^-1 -> M ->
-//                 clip -> positions and
-//                 the vertex stage of each corner (normals, colours, light,
-//                 camera matrices)
+//   k_bwd_pixels  one wavefront per 16x16 tile (4 pixels per lane): each
+//                 distinct primitive's setup and corner stage once, in LDS;
+//                 per pixel, shading backward (texture / specular gradients
+//                 summed per texel in LDS), the visibility backward (dL/dM^-1,
+//                 dL/dz_clip, per-corner varying cotangents) summed per
+//                 triangle in LDS, then one set of global atomics per
+//                 (tile, triangle) and per (tile, texel)
+//   k_bwd_tris    one thread per triangle, the touched ones packed together:
+//                 M^-1 -> M -> clip -> positions and the vertex stage of each
+//                 corner (normals, colours, light, camera matrices)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -391,7 +392,7 @@ struct BwdArgs {
   const float* d_zbuffer;
   const int32_t* prim_id;
   float* tacc;              // kAcc floats per triangle, zero between calls
-  int32_t* touched;         // per triangle: 1 once a pixel adds to it, zero between calls
+  int32_t* touched;         // per triangle: its scene + 1 once a pixel adds to it, zero between calls
   uint32_t* header;         // workspace header (kHdr* words)
   int32_t W, H, tiles_x, tiles_y, T, n_scenes;
   uint32_t T_m, ty_m;  // udiv31 by T and tiles_y
@@ -573,7 +574,7 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
         for (int k = 0; k < kAcc; ++k)
           if (vals[k] != 0.0f) atomicAdd(&acc[e_slot][k], vals[k]);
       } else {
-        a.touched[t] = 1;  // k_bwd_tris only visits touched triangles
+        a.touched[t] = s + 1;  // k_bwd_tris only visits touched triangles
         for (int k = 0; k < kAcc; ++k)
           if (vals[k] != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + k, vals[k]);
       }
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   if (lane < kBwdCap) {
     const int32_t t = tab[lane];
     if (t >= 0) {
-      a.touched[t] = 1;
+      a.touched[t] = s + 1;
       for (int k = 0; k < kAcc; ++k) {
         const float v = acc[lane][k];
         if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + k, v);
@@ -675,13 +676,12 @@ JR_DEV void vertex_bwd(const BwdArgs& a, const jr_scene& sc, int32_t slot, const
 // cleared only when `touched`.  Every lane computes its scene, so that a
 // wavefront of one scene reduces its camera and light gradients once.
 template <int SHADER>
-JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, bool touched) {
+JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
   float dW[16], dE[16], dLd[3], dLc[3];
   for (int q = 0; q < 16; ++q) dW[q] = dE[q] = 0.0f;
   for (int q = 0; q < 3; ++q) dLd[q] = dLc[q] = 0.0f;
   if (__ballot(touched) == 0) return;  // nothing to add for this wavefront
-  int32_t s = -1;
-  if (t >= 0) s = scene_of_tri_wave(a.scenes, a.n_scenes, t);
+  if (!touched) s = -1;
   if (touched) {
     float* A = a.tacc + (int64_t)t * kAcc;
     bool any = false;
@@ -769,13 +769,27 @@ JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, bool touched) {
   }
 }
 
-// One thread per triangle; only the touched ones read their accumulators,
-// which are then cleared with the flag for the next call.
+// One thread per triangle: the touched ones (flag = scene + 1) are packed to
+// the front of the workgroup, read their accumulators and clear them with the
+// flag for the next call; wavefronts left without one return at once.
 template <int SHADER>
 __global__ __launch_bounds__(256) void k_bwd_tris(BwdArgs a) {
+  __shared__ int2 items[256];
+  __shared__ int32_t wave_n[4];
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = t < a.in.n_tris;
-  bwd_tri<SHADER>(a, valid ? t : -1, valid && a.touched[t] != 0);
+  const int32_t flag = t < a.in.n_tris ? a.touched[t] : 0;
+  const uint64_t bal = __ballot(flag != 0);
+  if ((threadIdx.x & 63) == 0) wave_n[threadIdx.x >> 6] = __popcll(bal);
+  __syncthreads();
+  int32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) pos += wave_n[w];
+  if (flag != 0) items[pos] = make_int2(t, flag - 1);
+  __syncthreads();
+  const int32_t n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+  const bool has = (int32_t)threadIdx.x < n;
+  const int2 it = has ? items[threadIdx.x] : make_int2(-1, -1);
+  bwd_tri<SHADER>(a, it.x, it.y, has);
 }
 
 // Reads the workspace header: if it does not describe this workspace (new,
