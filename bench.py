@@ -243,25 +243,31 @@ def main() -> None:
     px = {"main": B * W * H, "shadow": B * smspec.width * smspec.height}
     tiles = {tag: p.n_tiles for tag, p in br.passes.items()}
 
+    # the shadow pass keeps 32-bit depth keys in its output buffer (a depth
+    # pass with an offset and no id outputs); the main pass 64-bit keys
+    key_b = {"shadow": 4, "main": 8}
+
     def alg_bytes(pass_: str, stage: str) -> int:
-        if stage == "clear":  # keys (8 B/pixel), tile counters, sample tables
-            return px[pass_] * 8 + tiles[pass_] * 8
+        if stage == "clear":  # key per pixel, tile counters
+            return px[pass_] * key_b[pass_] + tiles[pass_] * 8
         if stage == "setup":  # 3 slot ids (+3 position indices in the main
-            # pass), 3 positions read; 64 B record + 16 B tile range written
-            return n_tri * (12 + (12 if pass_ == "main" else 0) + 36 + 80)
+            # pass) and 3 positions read, one 4 B binning word written per
+            # triangle (records of the triangles that survive culling not
+            # counted: a lower bound)
+            return n_tri * (12 + (12 if pass_ == "main" else 0) + 36 + 4)
         if stage == "wide":  # the wide list and ranges of wide triangles
             return 0
         if stage == "scan":  # count read, offset written, per tile
             return tiles[pass_] * 8
-        if stage == "bin":  # tile range read per triangle, 4 B per binned pair
-            return n_tri * 16 + pairs[pass_] * 4
-        # raster: keys read (8 B/px), depth (+ colour) written, 52 B per
-        # binned pair (id + 48 B record), and per visible triangle of the main
-        # pass its record (64 B), face row (12 B) and three corners' indices
-        # and attributes (4 x 4 B indices + position, normal, uv: 48 B each)
+        if stage == "bin":  # binning word per triangle, 4 B per binned pair
+            return n_tri * 4 + pairs[pass_] * 4
+        # raster: keys read, depth (+ colour) written, 52 B per binned pair
+        # (id + 48 B record), and per visible triangle of the main pass its
+        # record (64 B), face row (12 B) and three corners' indices and
+        # attributes (4 x 4 B indices + position, normal, uv: 48 B each)
         out = px[pass_] * (4 if pass_ == "shadow" else 16)
         shade = n_visible * (64 + 12 + 3 * 48) if pass_ == "main" else 0
-        return px[pass_] * 8 + out + pairs[pass_] * 52 + shade
+        return px[pass_] * key_b[pass_] + out + pairs[pass_] * 52 + shade
 
     dpass, dstage = dom.split(".")
     dom_bytes = alg_bytes(dpass, dstage)
