@@ -405,12 +405,16 @@ struct BwdArgs {
 constexpr int kHdrMagicLo = 0, kHdrMagicHi = 1, kHdrTris = 2, kHdrFill = 3;
 constexpr uint32_t kMagicLo = 0x6a724277u, kMagicHi = 0x73706b31u;
 
-// One wavefront per 16x16 tile, lane l owning pixels (16 tx + l / 16 + 4 q,
-// 16 ty + l % 16), q < 4.  The tile's distinct primitives (up to kBwdCap)
-// get their setup (M^-1 columns and clip z) and, for the split shaders, their
-// corner stage computed once, by one lane each, into LDS; their per-triangle
-// cotangents are summed in LDS and added to the global accumulators once per
-// (tile, triangle).  Pixels of primitives beyond the table's capacity compute
+// One wavefront per 16x16 tile, lane l owning the 2x2 block of pixels
+// (16 tx + 2 (l / 8) + q / 2, 16 ty + 2 (l % 8) + q % 2), q < 4.  The tile's
+// distinct primitives (up to kBwdCap) get their setup (M^-1 columns and clip
+// z) and, for the split shaders, their corner stage computed once, by one
+// lane each, into LDS.  A lane sums the cotangents of consecutive pixels of
+// one triangle (and the gradients of consecutive pixels on one texel) in
+// registers and adds a run to the LDS sums only when it ends, so neighbouring
+// pixels, which mostly share both, cost one LDS atomic instead of four; the
+// LDS sums reach the global accumulators once per (tile, triangle) and per
+// (tile, texel).  Pixels of primitives beyond the table's capacity compute
 // their own and add directly.
 constexpr int kBwdTile = 16;
 constexpr int kBwdCap = 32;
@@ -433,12 +437,13 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   const int32_t s = udiv31(gt, a.T_m, a.T_sh);
   const int32_t tile = gt - s * a.T;
   const int32_t tx = udiv31(tile, a.ty_m, a.ty_sh), ty = tile - tx * a.tiles_y;
-  const int32_t j = ty * kBwdTile + lane % kBwdTile;
   const jr_scene& sc = a.scenes[s];
   const float* vp = sc.viewport;
-  auto pixel_of = [&](int q, int32_t* i_out) -> int32_t {
-    const int32_t i = tx * kBwdTile + lane / kBwdTile + 4 * q;
+  auto pixel_of = [&](int q, int32_t* i_out, int32_t* j_out) -> int32_t {
+    const int32_t i = tx * kBwdTile + 2 * (lane >> 3) + (q >> 1);
+    const int32_t j = ty * kBwdTile + 2 * (lane & 7) + (q & 1);
     *i_out = i;
+    *j_out = j;
     if (i >= a.W || j >= a.H) return -1;
     return a.prim_id[(int64_t)s * a.W * a.H + (int64_t)i * a.H + j];
   };
@@ -456,8 +461,8 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    int32_t i;
-    const int32_t t = pixel_of(q, &i);
+    int32_t i, j;
+    const int32_t t = pixel_of(q, &i, &j);
     if (t >= 0) (void)table_insert(tab, kBwdCap, t);
   }
   __syncthreads();
@@ -486,11 +491,36 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
 
   PixGrad g;
   for (int q = 0; q < S_COUNT; ++q) g.sg[q] = 0.0f;
+  // the lane's open runs: triangle (its table entry, or -1 beyond the table),
+  // texel and specular-map entry, with their sums
+  int32_t run_t = -1, run_slot = -1;
+  float run_v[kAcc];
+  for (int k = 0; k < kAcc; ++k) run_v[k] = 0.0f;
+  int64_t run_tex = -1, run_spec = -1;
+  int32_t run_tex_n = 0;
+  float run_tex_g[3] = {0.0f, 0.0f, 0.0f}, run_spec_g = 0.0f;
+  auto flush_tri = [&]() {
+    if (run_t < 0) return;
+    if (run_slot >= 0) {
+      for (int k = 0; k < kAcc; ++k)
+        if (run_v[k] != 0.0f) atomicAdd(&acc[run_slot][k], run_v[k]);
+    } else {
+      a.touched[run_t] = s + 1;  // k_bwd_tris only visits touched triangles
+      for (int k = 0; k < kAcc; ++k)
+        if (run_v[k] != 0.0f) atomicAdd(a.tacc + (int64_t)run_t * kAcc + k, run_v[k]);
+    }
+    for (int k = 0; k < kAcc; ++k) run_v[k] = 0.0f;
+  };
 #pragma unroll 1
   for (int q = 0; q < 4; ++q) {
-    int32_t i;
-    const int32_t t = pixel_of(q, &i);
-    const int32_t e_slot = t >= 0 ? table_insert(tab, kBwdCap, t) : -1;  // finds its entry
+    int32_t i, j;
+    const int32_t t = pixel_of(q, &i, &j);
+    if (t != run_t) {
+      flush_tri();
+      run_t = t;
+      run_slot = t >= 0 ? table_insert(tab, kBwdCap, t) : -1;  // finds its entry
+    }
+    const int32_t e_slot = run_slot;
     const int64_t pix = (int64_t)s * a.W * a.H + (int64_t)i * a.H + j;
     g.tex_at = -1;
     g.spec_at = -1;
@@ -570,18 +600,26 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
       g.sg[S_VP11] += -dy * y / vp[5];
       for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) vals[A_VAR + 3 * k + c] = g.dvar[k][c];
-      if (e_slot >= 0) {
-        for (int k = 0; k < kAcc; ++k)
-          if (vals[k] != 0.0f) atomicAdd(&acc[e_slot][k], vals[k]);
-      } else {
-        a.touched[t] = s + 1;  // k_bwd_tris only visits touched triangles
-        for (int k = 0; k < kAcc; ++k)
-          if (vals[k] != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + k, vals[k]);
-      }
+      for (int k = 0; k < kAcc; ++k) run_v[k] += vals[k];
     }
-    stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, g.tex_at, g.tex_n, g.tex_g);
-    stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, g.spec_at, 1, &g.spec_g);
+    // texel and specular-map runs
+    if (g.tex_at != run_tex) {
+      stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, run_tex, run_tex_n, run_tex_g);
+      run_tex = g.tex_at;
+      run_tex_n = g.tex_n;
+      run_tex_g[0] = run_tex_g[1] = run_tex_g[2] = 0.0f;
+    }
+    for (int c = 0; c < 3; ++c) run_tex_g[c] += g.tex_g[c];
+    if (g.spec_at != run_spec) {
+      stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, run_spec, 1, &run_spec_g);
+      run_spec = g.spec_at;
+      run_spec_g = 0.0f;
+    }
+    run_spec_g += g.spec_g;
   }
+  flush_tri();
+  stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, run_tex, run_tex_n, run_tex_g);
+  stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, run_spec, 1, &run_spec_g);
   __syncthreads();
   // one texel's channels: tex_c is the scene's (one scene per tile)
   flush_adds<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, sc.tex_c > 1 ? 3 : 1);
