@@ -103,7 +103,9 @@ typedef struct jr_inputs {
   const int32_t* tex_index_index;
   const float* colours;         /* [*,3] per-vertex colour (gouraud) */
   const int32_t* col_index;
-  const int32_t* faces;         /* [n_tris,3] slot ids: pipeline `face_indices` */
+  const int32_t* faces;         /* [n_tris,3] slot ids: pipeline `face_indices`; NULL:
+                                   the slots of row t are 3t, 3t+1, 3t+2 (a per-corner
+                                   flattening; needs n_slots >= 3 n_tris) */
   const int32_t* id_to_face;    /* darboux: [n_slots] face row per slot */
   const int32_t* face_slots;    /* darboux: [*,3] slot ids per face row */
   const float* textures;        /* flat diffuse maps */

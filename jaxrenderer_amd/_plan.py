@@ -59,6 +59,17 @@ def to_device(x, dev: torch.device, name: str) -> Optional[torch.Tensor]:
     return t.contiguous()
 
 
+def device_inputs(call: RasterCall, dev: torch.device) -> Dict[str, Optional[torch.Tensor]]:
+    """The call's inputs as device tensors.  A call whose face table is the
+    identity of a per-corner flattening (extras["identity_faces"]) passes no
+    table: the library then takes slots 3t, 3t+1, 3t+2 for row t without
+    reading them."""
+    inputs = {k: to_device(v, dev, k) for k, v in call.inputs.items()}
+    if call.extras.get("identity_faces"):
+        inputs["faces"] = None
+    return inputs
+
+
 def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         canvas: Optional[torch.Tensor] = None, prim_id: bool = False, bary: bool = False,
         display: bool = False, tile=_raster.DEFAULT_TILE, persistent: bool = False,
@@ -89,7 +100,7 @@ def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
             if display and canvas is not None else None)
     if disp is not None:
         out["display"] = disp
-    inputs = {k: to_device(v, dev, k) for k, v in call.inputs.items()}
+    inputs = device_inputs(call, dev)
     scenes = _raster.scenes_to_device(call.scenes, dev)
     _raster.forward(
         shader=call.shader, width=W, height=H, scenes=scenes, n_scenes=B, inputs=inputs,

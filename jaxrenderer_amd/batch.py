@@ -150,7 +150,8 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
         n_slots=3 * toff, n_tris=toff, shadow_shape=shadow_shape, init_buffers=True,
         zbuffer_default=float(zbuffer_default),
         colour_default=tuple(float(c) for c in np.asarray(colour_default, f32).reshape(-1)[:3]),
-        shadow_from="shadow" if shadow is not None else None, tag="main"))
+        shadow_from="shadow" if shadow is not None else None, tag="main",
+        extras={"identity_faces": True}))
     return calls
 
 
@@ -162,7 +163,7 @@ class _Pass:
         self.persistent = persistent
         self.tiles_per_group = 1
         self.pixels_per_lane = 1
-        self.inputs = {k: _plan.to_device(v, dev, k) for k, v in call.inputs.items()}
+        self.inputs = _plan.device_inputs(call, dev)
         self.scenes = _raster.scenes_to_device(call.scenes, dev)
         B, W, H = call.n_scenes, call.width, call.height
         self.zbuffer = torch.empty((B, W, H), dtype=torch.float32, device=dev)

@@ -751,7 +751,7 @@ JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
           dM[3 * p + q] = -s2;
         }
       for (int k = 0; k < 3; ++k) {
-        const int32_t slot = a.in.faces[3 * (int64_t)t + k];
+        const int32_t slot = face_slot(a.in, t, k);
         const float dclip[4] = {dM[3 * k + 0], dM[3 * k + 1], v[A_Z + k], dM[3 * k + 2]};
         const int32_t prow = attr_row(a.in.pos_index, slot);
         float p[3];
@@ -909,7 +909,9 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
       sh >= JR_SHADER_COUNT)
     return bfail(JR_E_UNSUPPORTED, "shader has no backward pass");
   if (!d_canvas && !d_zbuffer) return bfail(JR_E_INVALID, "no upstream gradient");
-  if (!in->positions || !in->faces) return bfail(JR_E_INVALID, "positions and faces are required");
+  if (!in->positions) return bfail(JR_E_INVALID, "positions are required");
+  if (!in->faces && (int64_t)in->n_slots < 3 * (int64_t)in->n_tris)
+    return bfail(JR_E_INVALID, "without faces, n_slots must be at least 3 n_tris");
   if (sh == JR_SHADER_GOURAUD && (!in->normals || !in->colours))
     return bfail(JR_E_INVALID, "gouraud needs normals and colours");
   if ((sh == JR_SHADER_GOURAUD_TEXTURE || sh == JR_SHADER_PHONG ||

@@ -143,12 +143,18 @@ JR_DEV int32_t table_insert(int32_t* tab, int32_t cap, int32_t w) {
   return -1;
 }
 
+// Slot of corner k of face row t: faces[t][k], or 3 t + k when faces is NULL
+// (the identity table of a per-corner flattening, not read from memory).
+JR_DEV int32_t face_slot(const jr_inputs& in, int32_t t, int k) {
+  return in.faces ? in.faces[3 * (int64_t)t + k] : 3 * t + k;
+}
+
 // The slots of face row w's corners.
 JR_DEV Corner corner_slots(const jr_inputs& in, int32_t w) {
   Corner c;
-  c.slot[0] = in.faces[3 * (int64_t)w + 0];
-  c.slot[1] = in.faces[3 * (int64_t)w + 1];
-  c.slot[2] = in.faces[3 * (int64_t)w + 2];
+  c.slot[0] = face_slot(in, w, 0);
+  c.slot[1] = face_slot(in, w, 1);
+  c.slot[2] = face_slot(in, w, 2);
   return c;
 }
 
@@ -167,7 +173,7 @@ struct TriSetup {
 JR_DEV void tri_corners(const jr_inputs& in, const jr_scene& sc, int32_t t, float clip[3][4]) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int32_t slot = in.faces[3 * (int64_t)t + k];
+    const int32_t slot = face_slot(in, t, k);
     if (in.clip) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) clip[k][q] = in.clip[4 * (int64_t)slot + q];

@@ -1275,8 +1275,9 @@ int32_t check_inputs(const jr_frame* fr, const jr_inputs* in, const jr_outputs* 
   if (in->n_tris < 0 || in->n_slots < 0) return fail(JR_E_INVALID, "negative sizes");
   if (!out->zbuffer) return fail(JR_E_INVALID, "zbuffer is required");
   if (in->n_tris == 0) return JR_OK;
-  if ((!in->positions && !in->clip) || !in->faces)
-    return fail(JR_E_INVALID, "positions (or clip) and faces are required");
+  if (!in->positions && !in->clip) return fail(JR_E_INVALID, "positions (or clip) are required");
+  if (!in->faces && (int64_t)in->n_slots < 3 * (int64_t)in->n_tris)
+    return fail(JR_E_INVALID, "without faces, n_slots must be at least 3 n_tris");
   if (in->clip && !(fr->shader == JR_SHADER_VISIBILITY || fr->shader == JR_SHADER_DEPTH))
     return fail(JR_E_INVALID, "clip input is only valid with the depth/visibility shaders");
   const int sh = fr->shader;

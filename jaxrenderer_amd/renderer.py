@@ -175,7 +175,7 @@ class Renderer:
             inputs=inputs, n_slots=3 * F, n_tris=F, shadow_shape=shadow_shape,
             init_buffers=init_buffers, zbuffer_default=float(zbuffer_default),
             colour_default=tuple(float(c) for c in np.asarray(colour_default, f32).reshape(-1)[:3]),
-            shadow_from=shadow_from, tag="main",
+            shadow_from=shadow_from, tag="main", extras={"identity_faces": True},
         ))
         return calls
 
