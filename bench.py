@@ -272,18 +272,23 @@ def main() -> None:
     dpass, dstage = dom.split(".")
     dom_bytes = alg_bytes(dpass, dstage)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
-    traffic = None
+    traffic = valu_busy = None
     tpath = os.path.join(ROOT, "profiles", "config3_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             measured = json.load(f).get("kernels", {})
         if dom in measured:
             traffic = measured[dom]["hbm_bytes"]
+            valu_busy = measured[dom].get("valu_busy")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dom, "kernel_ms": round(stages[dom], 4), "alg_bytes": int(dom_bytes),
                 "alg_bytes_by_kernel": {k: alg_bytes(*k.split(".")) for k in stages},
-                "visible_triangles": n_visible}
+                "visible_triangles": n_visible,
+                # share of the chip's VALU issue slots the same kernel used
+                # (rocprof counters, profiles/config3_traffic.json): the
+                # per-texel edge arithmetic, not HBM, bounds the raster
+                "valu_busy": valu_busy}
 
     res = {
         "metric": "frames/s at 84x84 batch 1024 (fwd, fwd+bwd) vs per-scene launch, 1 GPU",
