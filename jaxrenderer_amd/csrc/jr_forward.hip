@@ -1020,8 +1020,11 @@ JR_DEV int64_t xcd_item(const RasterArgs& a, int32_t x, int64_t k) {
   return (sl * kXcds + x) * a.f.T + (k - sl * a.f.T);
 }
 
+// At most 72 registers (7 wavefronts per SIMD): the shading instances would
+// otherwise take 95 and run 5 per SIMD; the occupancy pays for the few spills.
 template <int SHADER, int TW, int TH, int PPL>
-__global__ __launch_bounds__(TW* TH / PPL) void k_raster(RasterArgs a) {
+__global__ __launch_bounds__(TW* TH / PPL) __attribute__((amdgpu_waves_per_eu(7, 8)))
+void k_raster(RasterArgs a) {
   __shared__ float4 lds_rec[3 * TW * TH / PPL];
   __shared__ int32_t lds_id[TW * TH / PPL];
   int64_t gt = blockIdx.x;
