@@ -294,11 +294,18 @@ JR_DEV bool depth_bounded(const float4& r0, const float4& r1, const float4& r2,
 
 constexpr int kSetupThreads = 256;
 
+// Per-lane data of k_setup's two cooperative loops.  A live triangle is
+// small (its pixels are tested one per lane-step), or counted into at most
+// kWideTiles tiles (one tile test per lane-step), or neither; box is its pixel
+// box (tx0, ty0, height, scene) or its tile range (tx0, ty0, height in tiles,
+// scene | sane << 30) accordingly.
 struct SmallStage {
   float4 r[3][kSetupThreads];
-  int4 box[kSetupThreads];  // tx0, ty0, box height, scene
+  int4 box[kSetupThreads];
   int32_t tri[kSetupThreads];
-  int32_t off[kSetupThreads];  // exclusive prefix sum of the areas per wavefront
+  int16_t off[kSetupThreads];   // exclusive prefix sum of the areas per wavefront
+  int16_t toff[kSetupThreads];  // exclusive prefix sum of the tile counts
+  uint32_t tmask[kSetupThreads];  // tiles that passed the test (bit q)
 };
 
 // A triangle handed from the first half of k_setup to the second: its clip
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
   // Second half, live triangles: inverse, records, pixel range; small
   // triangles are resolved into the key buffer here, the others counted into
   // the tiles they may cover.
-  int32_t n_small = 0;
+  int32_t n_small = 0, n_tile = 0;
   if (valid) {
     const jr_scene& sc = scenes[s];
     tri_det(ts.clip, ts.M);
@@ -437,31 +444,62 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
       // counted by k_wide (one lane per tile)
       wide[atomicAdd(wide_count, 1)] = make_int2(t, s);
       masks[t] = kMaskWide;
+    } else if (n_t == 0) {
+      masks[t] = kMaskNone;
     } else {
-      // count the tiles that may hold a covered pixel; remember which ones
-      // (bit q: tile (tx0 + q / h, ty0 + q % h)) for k_bin
-      int32_t* cnt = counts + (int64_t)s * f.T;
-      const float* sxs = samples + (int64_t)s * (f.W + f.H);
-      uint32_t mask = 0u;
-      int q = 0;
-      for (int32_t tx = r.tx0; tx <= r.tx1; ++tx)
-        for (int32_t ty = r.ty0; ty <= r.ty1; ++ty, ++q)
-          if (tile_may_cover(r0, r1, r2, sane, sc.viewport, sxs, f, tx, ty)) {
-            atomicAdd(cnt + tx * f.tiles_y + ty, 1);
-            mask |= 1u << q;
-          }
-      masks[t] = mask;
+      // counted below, one (triangle, tile) pair per lane-step
+      n_tile = (int32_t)n_t;
+      sm.r[0][threadIdx.x] = r0;
+      sm.r[1][threadIdx.x] = r1;
+      sm.r[2][threadIdx.x] = r2;
+      sm.box[threadIdx.x] =
+          make_int4(r.tx0, r.ty0, r.ty1 - r.ty0 + 1, s | (sane ? (1 << 30) : 0));
     }
   }
-  int32_t incl = n_small;
+  int32_t incl = n_small, incl_t = n_tile;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const int32_t v = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += v;
+    const int32_t vt = __shfl_up(incl_t, d, 64);
+    if (lane >= d) {
+      incl += v;
+      incl_t += vt;
+    }
   }
   const int32_t total = __shfl(incl, 63, 64);
-  sm.off[threadIdx.x] = incl - n_small;
+  const int32_t total_t = __shfl(incl_t, 63, 64);
+  sm.off[threadIdx.x] = (int16_t)(incl - n_small);
+  sm.toff[threadIdx.x] = (int16_t)(incl_t - n_tile);
+  sm.tmask[threadIdx.x] = 0u;
   __syncthreads();
+  // the tile test of every (triangle, tile) pair of the wavefront's counted
+  // triangles: count the tiles that may hold a covered pixel and remember
+  // which ones (bit q: tile (tx0 + q / h, ty0 + q % h)) for k_bin
+  for (int32_t base = 0; base < total_t; base += 64) {
+    const int32_t item = base + lane;
+    if (item >= total_t) break;
+    int32_t lo = 0, hi = 63;  // last lane whose first item is <= item
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (sm.toff[wbase + mid] <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int o = wbase + lo;
+    const int4 box = sm.box[o];
+    const int32_t q = item - sm.toff[o];
+    const int32_t so = box.w & 0x3FFFFFFF;
+    const int32_t dq = div_small(q, box.z);
+    const int32_t tx = box.x + dq, ty = box.y + (q - dq * box.z);
+    if (tile_may_cover(sm.r[0][o], sm.r[1][o], sm.r[2][o], (box.w >> 30) != 0,
+                       scenes[so].viewport, samples + (int64_t)so * (f.W + f.H), f, tx, ty)) {
+      atomicAdd(counts + (int64_t)so * f.T + tx * f.tiles_y + ty, 1);
+      atomicOr(&sm.tmask[o], 1u << q);
+    }
+  }
+  __syncthreads();
+  if (n_tile > 0) masks[t] = sm.tmask[threadIdx.x];
   // the tile raster's exact per-pixel arithmetic for every (triangle, pixel)
   for (int32_t base = 0; base < total; base += 64) {
     const int32_t item = base + lane;

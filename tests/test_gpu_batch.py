@@ -63,3 +63,27 @@ def test_bin_and_raster_parts_equal_whole_pass():
         br.passes[tag].launch(_hip.PART_RASTER)
     torch.cuda.synchronize()
     _same(want, _snapshot(br))
+
+
+def test_batch_backward_without_face_table_matches_explicit_table():
+    """The batch's main pass passes no face table (identity slots); its
+    gradients must equal those of the same step with the explicit table (up
+    to the order of the atomic sums)."""
+    br = _renderer(n=16, seed=8)
+    br.enable_backward()
+    out = br.render()
+    target = torch.rand_like(out["canvas"])
+    torch.cuda.synchronize()
+    main = br.passes["main"]
+    assert main.inputs["faces"] is None
+    got = {k: v.clone() for k, v in br.train_step(target).items()}
+    torch.cuda.synchronize()
+    n = main.call.n_tris
+    main.inputs["faces"] = torch.arange(3 * n, dtype=torch.int32, device=target.device).view(n, 3)
+    want = br.train_step(target)
+    torch.cuda.synchronize()
+    for k in want:
+        a, b = got[k].double(), want[k].double()
+        assert torch.isfinite(b).all(), k
+        scale = max(float(b.norm()), 1e-6)
+        assert float((a - b).norm()) <= 1e-5 * scale, k
