@@ -646,29 +646,82 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(
 }
 
 // Stage 3: write triangle ids into the tile lists.
+// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
+// each wavefront of the workgroup works on a tile of its own, a wavefront
+// barrier (a wavefront's LDS operations complete in issue order, so keeping
+// the compiler from moving memory operations across the point is enough).
+template <bool WAVE>
+JR_DEV void tile_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// One (triangle, tile) pair per lane-step: the wavefront's binned triangles'
+// set mask bits are numbered by a prefix sum of their counts and spread over
+// the lanes, so every step issues up to 64 independent slot claims instead of
+// each lane claiming its tiles one after another.  All staging is
+// wavefront-local.
 __global__ __launch_bounds__(256) void k_bin(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     const float4* __restrict__ rec, const Rect* __restrict__ rects,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
     int32_t* __restrict__ bins, const uint32_t* __restrict__ masks,
     const int32_t* __restrict__ wave_scene) {
+  __shared__ int4 own[256];  // tile range (tx0, ty0, height in tiles) and s * T
+  __shared__ int32_t own_t[256];
+  __shared__ uint32_t own_m[256];
+  __shared__ int32_t own_off[256];  // exclusive prefix sum of the bit counts
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(t & ~63) >= in.n_tris) return;
+  const int lane = threadIdx.x & 63;
+  const int wbase = threadIdx.x & ~63;
   const int32_t s = scene_of_wave(scenes, f.n_scenes, wave_scene, t, in.n_tris);
-  if (t >= in.n_tris) return;
-  uint32_t mask = masks[t];  // the tiles k_setup counted
-  if (mask == kMaskNone || mask == kMaskWide) return;  // nothing, or k_wide's
-  const Rect r = rects[t];
-  const int64_t tbase = (int64_t)s * f.T;
-  const int32_t h = r.ty1 - r.ty0 + 1;
-  while (mask) {
-    const int q = __ffs(mask) - 1;
-    mask &= mask - 1u;
-    const int32_t dq = div_small(q, h);
-    const int32_t tx = r.tx0 + dq, ty = r.ty0 + (q - dq * h);
-    const int64_t gt = tbase + tx * f.tiles_y + ty;
+  int32_t n_b = 0;
+  if (t < in.n_tris) {
+    const uint32_t mask = masks[t];  // the tiles k_setup counted
+    if (mask != kMaskNone && mask != kMaskWide) {  // (k_wide bins the wide ones)
+      const Rect r = rects[t];
+      own[threadIdx.x] = make_int4(r.tx0, r.ty0, r.ty1 - r.ty0 + 1, s * f.T);
+      own_t[threadIdx.x] = t;
+      own_m[threadIdx.x] = mask;
+      n_b = __popc(mask);
+    }
+  }
+  int32_t incl = n_b;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
+  }
+  const int32_t total = __shfl(incl, 63, 64);
+  own_off[threadIdx.x] = incl - n_b;
+  tile_sync<true>();
+  for (int32_t base = 0; base < total; base += 64) {
+    const int32_t item = base + lane;
+    if (item >= total) break;
+    int32_t lo = 0, hi = 63;  // last lane whose first item is <= item
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (own_off[wbase + mid] <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int o = wbase + lo;
+    uint32_t m = own_m[o];
+    for (int32_t k = item - own_off[o]; k > 0; --k) m &= m - 1u;  // k-th set bit
+    const int q = __ffs(m) - 1;
+    const int4 box = own[o];
+    const int32_t dq = div_small(q, box.z);
+    const int32_t tx = box.x + dq, ty = box.y + (q - dq * box.z);
+    const int64_t gt = (int64_t)box.w + tx * f.tiles_y + ty;
     const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
-    if (pos < f.capacity) bins[pos] = t;
+    if (pos < f.capacity) bins[pos] = own_t[o];
   }
 }
 
@@ -710,20 +763,6 @@ struct RasterArgs {
 
 // Chooser fallback, shading, merge and stores of pixel (i, j) of scene s
 // whose running minimum over all candidates is (best_key, best_id).
-// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
-// each wavefront of the workgroup works on a tile of its own, a wavefront
-// barrier (a wavefront's LDS operations complete in issue order, so keeping
-// the compiler from moving memory operations across the point is enough).
-template <bool WAVE>
-JR_DEV void tile_sync() {
-  if (WAVE) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  } else {
-    __syncthreads();
-  }
-}
 
 // The primitive a pixel shows: the chooser's winner, or, when no candidate
 // has a finite (or NaN) depth, argmin's index 0 of the scene
