@@ -940,6 +940,8 @@ JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t 
     }
   }
   tile_sync<WAVE>();
+  float l[3];
+  V::scene_terms(sc, l);
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     if (!inside[q]) continue;
@@ -953,7 +955,7 @@ JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t 
       } else {
         V::corners(a.in, sc, corner_slots(a.in, h[q].w), v);
       }
-      keep = V::pixel(a.in, sc, v, h[q].b, (h[q].fl & kFront) != 0, col, s);
+      keep = V::pixel(a.in, sc, v, l, h[q].b, (h[q].fl & kFront) != 0, col, s);
     }
     store_pixel(a, s, pi[q], pj[q], h[q], keep, col);
   }

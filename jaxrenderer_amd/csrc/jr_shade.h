@@ -98,10 +98,11 @@ JR_DEV float shadow_get(const float* __restrict__ map, int32_t sw, int32_t sh,
 // Phong reflection fragment core shared by the two reflection shaders
 // (shaders/phong_reflection.py:154-235, phong_reflection_shadow.py:173-271).
 // Computes tex colour, diffuse scalar and specular scalar.
+// l: normalise(light_dir_eye), the same for every fragment of a scene.
 JR_DEV void reflection_terms(const jr_inputs& in, const jr_scene& sc,
                              const float n_i[3], const float uv[2],
-                             int32_t tex_index, float tex[3], float* diffuse,
-                             float* specular) {
+                             int32_t tex_index, const float l[3], float tex[3],
+                             float* diffuse, float* specular) {
   // texture_shape[texture_index] and MergedModel.uv_repeat
   // (renderer/model.py:303-339)
   const int32_t row = index_clamp(tex_index, sc.texshape_count);
@@ -123,9 +124,8 @@ JR_DEV void reflection_terms(const jr_inputs& in, const jr_scene& sc,
   const int32_t vs = index_clamp(v0, sc.spec_h);
   const float spec_exp = in.specular_maps[sc.spec_base + (int64_t)us * sc.spec_h + vs];
 
-  float n[3], l[3];
+  float n[3];
   normalise3(n_i, n);
-  normalise3(sc.light_dir_eye, l);
   const float d = dot3(n, l);
   *diffuse = max_nan(d, 0.0f);
   const float two_d = 2.0f * d;
@@ -372,8 +372,13 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
     }
     v[15] = __int_as_float(in.tex_index[attr_row(in.tex_index_index, c.slot[0])]);
   }
+  // per-scene terms of the pixel stage, computed once per tile
+  static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
+    normalise3(sc.light_dir_eye, l);
+  }
   static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
-                           const float b[3], bool front, float col[3], int) {
+                           const float l[3], const float b[3], bool front, float col[3],
+                           int) {
     const float* ne = v;
     const float* uvv = v + 9;
     const int32_t ti = __float_as_int(v[15]);
@@ -383,7 +388,7 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
     uv[0] = interp(b, uvv[0], uvv[2], uvv[4]);
     uv[1] = interp(b, uvv[1], uvv[3], uvv[5]);
     float tex[3], diff, spec;
-    reflection_terms(in, sc, n, uv, ti, tex, &diff, &spec);
+    reflection_terms(in, sc, n, uv, ti, l, tex, &diff, &spec);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       const float k = sc.diffuse[ch] * diff + sc.specular[ch] * spec;
@@ -414,8 +419,12 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
     }
     v[27] = __int_as_float(in.tex_index[attr_row(in.tex_index_index, c.slot[0])]);
   }
+  static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
+    normalise3(sc.light_dir_eye, l);
+  }
   static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
-                           const float b[3], bool front, float col[3], int scene) {
+                           const float l[3], const float b[3], bool front, float col[3],
+                           int scene) {
     const float* ne = v;
     const float* uvv = v + 9;
     const float* scd = v + 15;
@@ -435,7 +444,7 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
     const float sd = shadow_get(map, in.shadow_w, in.shadow_h, sx, sy);
     const bool lit = sz <= sd;
     float tex[3], diff, spec;
-    reflection_terms(in, sc, n, uv, ti, tex, &diff, &spec);
+    reflection_terms(in, sc, n, uv, ti, l, tex, &diff, &spec);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       const float shadow = lit ? 1.0f : 1.0f - sc.shadow_strength[ch];
@@ -451,9 +460,10 @@ struct Shade<JR_SHADER_PHONG_REFLECTION> {
   static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                          const float b[3], bool front, float col[3], int scene) {
     using V = Varyings<JR_SHADER_PHONG_REFLECTION>;
-    float v[V::N];
+    float v[V::N], l[3];
     V::corners(in, sc, c, v);
-    return V::pixel(in, sc, v, b, front, col, scene);
+    V::scene_terms(sc, l);
+    return V::pixel(in, sc, v, l, b, front, col, scene);
   }
 };
 
@@ -462,9 +472,10 @@ struct Shade<JR_SHADER_PHONG_REFLECTION_SHADOW> {
   static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                          const float b[3], bool front, float col[3], int scene) {
     using V = Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW>;
-    float v[V::N];
+    float v[V::N], l[3];
     V::corners(in, sc, c, v);
-    return V::pixel(in, sc, v, b, front, col, scene);
+    V::scene_terms(sc, l);
+    return V::pixel(in, sc, v, l, b, front, col, scene);
   }
 };
 
