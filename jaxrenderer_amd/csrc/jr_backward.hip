@@ -248,7 +248,7 @@ struct ReflectionBwd {
                            PixGrad& g, int scene) {
     const float* nev = vary;      // eye-space normal of corner k: nev[3k..3k+2]
     const float* uvv = vary + 9;  // uv of corner k: uvv[2k..2k+1]
-    const int32_t ti = __float_as_int(vary[V::N - 1]);
+    const int32_t ti = __float_as_int(vary[V::kTi]);
     float n[3], uv[2];
     for (int q = 0; q < 3; ++q) n[q] = interp(b, nev[q], nev[3 + q], nev[6 + q]);
     uv[0] = interp(b, uvv[0], uvv[2], uvv[4]);
@@ -264,9 +264,7 @@ struct ReflectionBwd {
                                           sv[1] / sv[3]);
     }
     // forward of reflection_terms with its intermediates
-    const int32_t row = index_clamp(ti, sc.texshape_count);
-    const int32_t s0 = in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 0];
-    const int32_t s1 = in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 1];
+    const int32_t s0 = __float_as_int(vary[V::kShape]), s1 = __float_as_int(vary[V::kShape + 1]);
     float f0 = modf_frac(uv[0]), f1 = modf_frac(uv[1]);
     f0 = (f0 < 0.0f) ? f0 + 1.0f : f0;
     f1 = (f1 < 0.0f) ? f1 + 1.0f : f1;

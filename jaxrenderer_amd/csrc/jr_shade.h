@@ -99,15 +99,20 @@ JR_DEV float shadow_get(const float* __restrict__ map, int32_t sw, int32_t sh,
 // (shaders/phong_reflection.py:154-235, phong_reflection_shadow.py:173-271).
 // Computes tex colour, diffuse scalar and specular scalar.
 // l: normalise(light_dir_eye), the same for every fragment of a scene.
+// texture_shape[texture_index] (the gather clamps the row), per triangle.
+JR_DEV void texture_shape(const jr_inputs& in, const jr_scene& sc, int32_t tex_index,
+                          float* s0, float* s1) {
+  const int32_t row = index_clamp(tex_index, sc.texshape_count);
+  *s0 = __int_as_float(in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 0]);
+  *s1 = __int_as_float(in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 1]);
+}
+
+// s0, s1: texture_shape[texture_index] (texture_shape above).
 JR_DEV void reflection_terms(const jr_inputs& in, const jr_scene& sc,
                              const float n_i[3], const float uv[2],
-                             int32_t tex_index, const float l[3], float tex[3],
-                             float* diffuse, float* specular) {
-  // texture_shape[texture_index] and MergedModel.uv_repeat
-  // (renderer/model.py:303-339)
-  const int32_t row = index_clamp(tex_index, sc.texshape_count);
-  const int32_t s0 = in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 0];
-  const int32_t s1 = in.texture_shapes[2 * (int64_t)(sc.texshape_base + row) + 1];
+                             int32_t tex_index, int32_t s0, int32_t s1, const float l[3],
+                             float tex[3], float* diffuse, float* specular) {
+  // MergedModel.uv_repeat (renderer/model.py:303-339)
   float f0 = modf_frac(uv[0]);
   float f1 = modf_frac(uv[1]);
   f0 = (f0 < 0.0f) ? f0 + 1.0f : f0;
@@ -362,7 +367,9 @@ struct Varyings {
 // uv; the texture index of corner 0.
 template <>
 struct Varyings<JR_SHADER_PHONG_REFLECTION> {
-  static constexpr int N = 16;
+  static constexpr int N = 18;
+  static constexpr int kTi = 15;     // texture index of corner 0
+  static constexpr int kShape = 16;  // its texture_shape row (2 ints)
   static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                              float v[N]) {
 #pragma unroll
@@ -370,7 +377,9 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
       eye_normal(in, sc, c.slot[k], v + 3 * k);
       load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
     }
-    v[15] = __int_as_float(in.tex_index[attr_row(in.tex_index_index, c.slot[0])]);
+    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+    v[kTi] = __int_as_float(ti);
+    texture_shape(in, sc, ti, v + kShape, v + kShape + 1);
   }
   // per-scene terms of the pixel stage, computed once per tile
   static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
@@ -381,14 +390,15 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
                            int) {
     const float* ne = v;
     const float* uvv = v + 9;
-    const int32_t ti = __float_as_int(v[15]);
+    const int32_t ti = __float_as_int(v[kTi]);
     float n[3], uv[2];
 #pragma unroll
     for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[q], ne[3 + q], ne[6 + q]);
     uv[0] = interp(b, uvv[0], uvv[2], uvv[4]);
     uv[1] = interp(b, uvv[1], uvv[3], uvv[5]);
     float tex[3], diff, spec;
-    reflection_terms(in, sc, n, uv, ti, l, tex, &diff, &spec);
+    reflection_terms(in, sc, n, uv, ti, __float_as_int(v[kShape]), __float_as_int(v[kShape + 1]), l,
+                     tex, &diff, &spec);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       const float k = sc.diffuse[ch] * diff + sc.specular[ch] * spec;
@@ -402,7 +412,9 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
 // homogeneous-normalised shadow-camera clip position.
 template <>
 struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
-  static constexpr int N = 28;
+  static constexpr int N = 30;
+  static constexpr int kTi = 27;
+  static constexpr int kShape = 28;
   static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                              float v[N]) {
 #pragma unroll
@@ -417,7 +429,9 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
       v[15 + 4 * k + 2] = h[2] / h[3];
       v[15 + 4 * k + 3] = h[3] / h[3];
     }
-    v[27] = __int_as_float(in.tex_index[attr_row(in.tex_index_index, c.slot[0])]);
+    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+    v[kTi] = __int_as_float(ti);
+    texture_shape(in, sc, ti, v + kShape, v + kShape + 1);
   }
   static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
     normalise3(sc.light_dir_eye, l);
@@ -428,7 +442,7 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
     const float* ne = v;
     const float* uvv = v + 9;
     const float* scd = v + 15;
-    const int32_t ti = __float_as_int(v[27]);
+    const int32_t ti = __float_as_int(v[kTi]);
     float n[3], uv[2], s4[4];
 #pragma unroll
     for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[q], ne[3 + q], ne[6 + q]);
@@ -444,7 +458,8 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
     const float sd = shadow_get(map, in.shadow_w, in.shadow_h, sx, sy);
     const bool lit = sz <= sd;
     float tex[3], diff, spec;
-    reflection_terms(in, sc, n, uv, ti, l, tex, &diff, &spec);
+    reflection_terms(in, sc, n, uv, ti, __float_as_int(v[kShape]), __float_as_int(v[kShape + 1]), l,
+                     tex, &diff, &spec);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       const float shadow = lit ? 1.0f : 1.0f - sc.shadow_strength[ch];
