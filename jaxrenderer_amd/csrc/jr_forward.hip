@@ -538,14 +538,22 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
 }
 
 // Counts (BIN = false) or bins (BIN = true) the wide triangles: one wavefront
-// per triangle, its lanes striding over the triangle's tiles.
+// per triangle, its lanes striding over the triangle's tiles.  The counting
+// pass also keeps each triangle's tile-test results as a bitmap (one word per
+// 32 tiles), so the binning pass reads bits instead of testing the tiles
+// again: entry w of the wide list owns the fixed words [w * per, (w + 1) *
+// per), per = ceil(T / 32), while they fit in the pool; later entries test
+// again.
+constexpr int32_t kWideBitWords = 1 << 20;
+
 template <bool BIN>
 __global__ __launch_bounds__(256) void k_wide(
     const jr_scene* __restrict__ scenes, Frame f, const float4* __restrict__ rec,
     const Rect* __restrict__ rects, const int2* __restrict__ wide,
     const int32_t* __restrict__ wide_count, int32_t* __restrict__ counts,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ bins, const float* __restrict__ samples) {
+    int32_t* __restrict__ bins, const float* __restrict__ samples,
+    uint32_t* __restrict__ wbits) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -554,23 +562,46 @@ __global__ __launch_bounds__(256) void k_wide(
     const int2 ts = wide[w];  // (triangle, scene)
     const int32_t t = ts.x, s = ts.y;
     const Rect r = rects[t];
-    const float* vp = scenes[s].viewport;
-    const float4 r0 = rec[4 * (int64_t)t], r1 = rec[4 * (int64_t)t + 1],
-                 r2 = rec[4 * (int64_t)t + 2];
-    const bool sane = (__float_as_uint(rec[4 * (int64_t)t + 3].y) & kSane) != 0;
     const int32_t h = r.ty1 - r.ty0 + 1;
     const int32_t n_t = (r.tx1 - r.tx0 + 1) * h;
     const int64_t tbase = (int64_t)s * f.T;
-    for (int32_t q = lane; q < n_t; q += 64) {
-      const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
-      if (!tile_may_cover(r0, r1, r2, sane, vp, samples + (int64_t)s * (f.W + f.H), f, tx, ty))
-        continue;
-      const int64_t gt = tbase + tx * f.tiles_y + ty;
-      if (BIN) {
-        const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
-        if (pos < f.capacity) bins[pos] = t;
-      } else {
-        atomicAdd(counts + gt, 1);
+    const int32_t per = (f.T + 31) >> 5;
+    const int32_t base = (w < kWideBitWords / per) ? (int32_t)w * per : -1;
+    float4 r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), r1 = r0, r2 = r0;
+    bool sane = false;
+    const float* vp = scenes[s].viewport;
+    if (!BIN || base < 0) {
+      r0 = rec[4 * (int64_t)t];
+      r1 = rec[4 * (int64_t)t + 1];
+      r2 = rec[4 * (int64_t)t + 2];
+      sane = (__float_as_uint(rec[4 * (int64_t)t + 3].y) & kSane) != 0;
+    }
+    for (int32_t q0 = 0; q0 < n_t; q0 += 64) {
+      const int32_t q = q0 + lane;
+      bool pass = false;
+      if (q < n_t) {
+        const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
+        if (BIN && base >= 0)
+          pass = ((wbits[base + (q >> 5)] >> (q & 31)) & 1u) != 0u;
+        else
+          pass = tile_may_cover(r0, r1, r2, sane, vp, samples + (int64_t)s * (f.W + f.H), f,
+                                tx, ty);
+        if (pass) {
+          const int64_t gt = tbase + tx * f.tiles_y + ty;
+          if (BIN) {
+            const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
+            if (pos < f.capacity) bins[pos] = t;
+          } else {
+            atomicAdd(counts + gt, 1);
+          }
+        }
+      }
+      if (!BIN) {
+        const uint64_t b = __ballot(pass);
+        if (base >= 0 && lane == 0) {
+          wbits[base + (q0 >> 5)] = (uint32_t)b;
+          if (q0 + 32 < n_t) wbits[base + (q0 >> 5) + 1] = (uint32_t)(b >> 32);
+        }
       }
     }
   }
@@ -1218,7 +1249,7 @@ int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
   int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, samples, masks,
-      wave_scene, bins, total;
+      wave_scene, wbits, bins, total;
   int64_t n_tiles, nblk, n_pix;
 };
 
@@ -1249,6 +1280,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.samples = o;    o += align_up(4 * (int64_t)(fr->width + fr->height) * fr->n_scenes);
   L.masks = o;      o += align_up(4 * (int64_t)std::max(n_tris, 1));
   L.wave_scene = o; o += align_up(4 * (((int64_t)std::max(n_tris, 1) + 63) / 64));
+  L.wbits = o;      o += align_up(4 * (int64_t)kWideBitWords);
   L.bins = o;       o += align_up(4 * std::max<int64_t>(fr->bin_capacity, 1));
   L.total = o;
   return L;
@@ -1459,6 +1491,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   uint32_t* masks = reinterpret_cast<uint32_t*>(ws + L.masks);
   int32_t* wave_scene = reinterpret_cast<int32_t*>(ws + L.wave_scene);
   int32_t* wide_count = misc + 2;
+  uint32_t* wbits = reinterpret_cast<uint32_t*>(ws + L.wbits);
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
   StageTimer timer(st, fr->part == JR_PART_ALL);
@@ -1484,7 +1517,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   timer.mark(2);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
-                       rects, wide, wide_count, counts, offsets, cursor, bins, samples);
+                       rects, wide, wide_count, counts, offsets, cursor, bins, samples, wbits);
   timer.mark(3);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)std::max<int64_t>(L.nblk, 1)),
                      dim3(kScanThreads), 0, st, counts, offsets, block_sums, L.n_tiles);
@@ -1497,7 +1530,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
                        scenes, f, rec, rects, offsets, cursor, bins, masks, wave_scene);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
-                       rects, wide, wide_count, counts, offsets, cursor, bins, samples);
+                       rects, wide, wide_count, counts, offsets, cursor, bins, samples, wbits);
   }
   timer.mark(5);
   }
