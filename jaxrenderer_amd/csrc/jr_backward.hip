@@ -347,10 +347,25 @@ struct ShadeBwd<JR_SHADER_PHONG_REFLECTION> : ReflectionBwd<false> {};
 template <>
 struct ShadeBwd<JR_SHADER_PHONG_REFLECTION_SHADOW> : ReflectionBwd<true> {};
 
+// v moved across lanes by the DPP pattern CTRL (lanes without a source get 0).
+template <int CTRL>
+JR_DEV float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Sum over the 64 lanes of the wavefront, returned to every lane: lane pairs
+// and quads (quad_perm [1,0,3,2], [2,3,0,1]), rows of 16 (row_ror 4, 8), then
+// rows 0-1 into 1 and 2-3 into 3 (row_bcast 15) and rows 0-1 into 2-3
+// (row_bcast 31), so lane 63 holds the total; all register-to-register, no
+// LDS.  Call with every lane active.
 JR_DEV float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v += dpp_move<0xb1>(v);
+  v += dpp_move<0x4e>(v);
+  v += dpp_move<0x124>(v);
+  v += dpp_move<0x128>(v);
+  v += dpp_move<0x142>(v);
+  v += dpp_move<0x143>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // d[at + c] += v[c] (c < n), summed first in an LDS table of (index, sum)
