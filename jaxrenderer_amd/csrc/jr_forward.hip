@@ -292,6 +292,54 @@ JR_DEV bool depth_bounded(const float4& r0, const float4& r1, const float4& r2,
   return b < 1e37f;
 }
 
+// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
+// each wavefront of the workgroup works on a tile of its own, a wavefront
+// barrier (a wavefront's LDS operations complete in issue order, so keeping
+// the compiler from moving memory operations across the point is enough).
+template <bool WAVE>
+JR_DEV void tile_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// max(v, v moved by the DPP pattern CTRL into the rows of ROW_MASK); lanes
+// without a source compare against -1.
+template <int CTRL, int ROW_MASK>
+JR_DEV int32_t dpp_max_step(int32_t v) {
+  return max(v, __builtin_amdgcn_update_dpp(-1, v, CTRL, ROW_MASK, 0xf, false));
+}
+
+// For lane k owning items [first_k, first_k + count_k) (first ascending over
+// the lanes, count 0 allowed): the owner of item base + lane, i.e. the last
+// lane whose first item is <= it.  Lanes whose range starts inside the window
+// mark their start in the wavefront's 64-entry LDS scratch `mark`; lane 0
+// starts from the last lane starting at or before base (a ballot), and an
+// inclusive prefix maximum over the lanes (DPP: row shifts 1, 2, 4, 8, then
+// the row broadcasts 15 and 31) carries each owner forward.  Call with every
+// lane of the wavefront active.
+JR_DEV int32_t wave_owner(int32_t first, int32_t count, int32_t base, int32_t* mark,
+                          int lane) {
+  mark[lane] = -1;
+  tile_sync<true>();
+  if (count > 0 && first > base && first < base + 64) mark[first - base] = lane;
+  tile_sync<true>();
+  int32_t v = mark[lane];
+  const uint64_t le = __ballot(first <= base);
+  if (lane == 0) v = 63 - __clzll((long long)le);
+  v = dpp_max_step<0x111, 0xf>(v);
+  v = dpp_max_step<0x112, 0xf>(v);
+  v = dpp_max_step<0x114, 0xf>(v);
+  v = dpp_max_step<0x118, 0xf>(v);
+  v = dpp_max_step<0x142, 0xa>(v);
+  v = dpp_max_step<0x143, 0xc>(v);
+  return v;
+}
+
 constexpr int kSetupThreads = 256;
 
 // Per-lane data of k_setup's two cooperative loops.  A live triangle is
@@ -329,6 +377,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene) {
   __shared__ SetupStage stage;
   __shared__ int32_t wave_live[kSetupThreads / 64];
+  __shared__ int32_t mark[kSetupThreads];  // wave_owner scratch, 64 per wavefront
   SmallStage& sm = stage.sm;
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
@@ -477,16 +526,8 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
   // which ones (bit q: tile (tx0 + q / h, ty0 + q % h)) for k_bin
   for (int32_t base = 0; base < total_t; base += 64) {
     const int32_t item = base + lane;
-    if (item >= total_t) break;
-    int32_t lo = 0, hi = 63;  // last lane whose first item is <= item
-    while (lo < hi) {
-      const int32_t mid = (lo + hi + 1) >> 1;
-      if (sm.toff[wbase + mid] <= item)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const int o = wbase + lo;
+    const int o = wbase + wave_owner(incl_t - n_tile, n_tile, base, mark + wbase, lane);
+    if (item >= total_t) continue;
     const int4 box = sm.box[o];
     const int32_t q = item - sm.toff[o];
     const int32_t so = box.w & 0x3FFFFFFF;
@@ -503,16 +544,8 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
   // the tile raster's exact per-pixel arithmetic for every (triangle, pixel)
   for (int32_t base = 0; base < total; base += 64) {
     const int32_t item = base + lane;
-    if (item >= total) break;
-    int32_t lo = 0, hi = 63;  // last lane whose first item is <= item
-    while (lo < hi) {
-      const int32_t mid = (lo + hi + 1) >> 1;
-      if (sm.off[wbase + mid] <= item)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const int o = wbase + lo;
+    const int o = wbase + wave_owner(incl - n_small, n_small, base, mark + wbase, lane);
+    if (item >= total) continue;
     const int4 box = sm.box[o];
     const int32_t q = item - sm.off[o];
     // q / h for q < 64, h <= 64: a float estimate, corrected to the exact quotient
@@ -677,21 +710,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(
 }
 
 // Stage 3: write triangle ids into the tile lists.
-// Orders a tile's LDS staging against its use: a workgroup barrier, or, when
-// each wavefront of the workgroup works on a tile of its own, a wavefront
-// barrier (a wavefront's LDS operations complete in issue order, so keeping
-// the compiler from moving memory operations across the point is enough).
-template <bool WAVE>
-JR_DEV void tile_sync() {
-  if (WAVE) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  } else {
-    __syncthreads();
-  }
-}
-
 // One (triangle, tile) pair per lane-step: the wavefront's binned triangles'
 // set mask bits are numbered by a prefix sum of their counts and spread over
 // the lanes, so every step issues up to 64 independent slot claims instead of
@@ -707,6 +725,7 @@ __global__ __launch_bounds__(256) void k_bin(
   __shared__ int32_t own_t[256];
   __shared__ uint32_t own_m[256];
   __shared__ int32_t own_off[256];  // exclusive prefix sum of the bit counts
+  __shared__ int32_t mark[256];     // wave_owner scratch, 64 per wavefront
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(t & ~63) >= in.n_tris) return;
   const int lane = threadIdx.x & 63;
@@ -734,16 +753,8 @@ __global__ __launch_bounds__(256) void k_bin(
   tile_sync<true>();
   for (int32_t base = 0; base < total; base += 64) {
     const int32_t item = base + lane;
-    if (item >= total) break;
-    int32_t lo = 0, hi = 63;  // last lane whose first item is <= item
-    while (lo < hi) {
-      const int32_t mid = (lo + hi + 1) >> 1;
-      if (own_off[wbase + mid] <= item)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const int o = wbase + lo;
+    const int o = wbase + wave_owner(incl - n_b, n_b, base, mark + wbase, lane);
+    if (item >= total) continue;
     uint32_t m = own_m[o];
     for (int32_t k = item - own_off[o]; k > 0; --k) m &= m - 1u;  // k-th set bit
     const int q = __ffs(m) - 1;
