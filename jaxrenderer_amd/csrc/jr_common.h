@@ -85,32 +85,9 @@ JR_DEV int32_t div_small(int32_t q, int32_t h) {
   return (q * ((256 + h - 1) / h)) >> 8;
 }
 
-JR_DEV int32_t scene_of_tri(const jr_scene* __restrict__ scenes, int32_t n,
-                            int32_t t) {
-  int32_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (scenes[mid].tri_begin <= t)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-// scene_of_tri for a wavefront whose triangles mostly share one scene: the
-// first active lane's scene is found with wavefront-uniform (scalar) loads,
-// and only lanes outside its triangle range search on their own.
-JR_DEV int32_t scene_of_tri_wave(const jr_scene* __restrict__ scenes, int32_t n,
-                                 int32_t t) {
-  const int32_t t0 = __builtin_amdgcn_readfirstlane(t);
-  const int32_t s0 = __builtin_amdgcn_readfirstlane(scene_of_tri(scenes, n, t0));
-  if (t >= scenes[s0].tri_begin && t < scenes[s0].tri_end) return s0;
-  return scene_of_tri(scenes, n, t);
-}
-
-// scene_of_tri for the wavefront of triangles [t0, t0 + 64), t0 = t & ~63,
-// from a table holding the scene of every wavefront's first triangle
+// The scene of triangle t (the last scene whose tri_begin <= t; scene ranges
+// ascend) for the wavefront of triangles [t0, t0 + 64), t0 = t & ~63, from a
+// table holding the scene of every wavefront's first triangle
 // (k_clear fills it): one scalar load, then a wavefront-uniform walk over the
 // scenes that begin inside the wavefront's range.  Call from converged code
 // of a kernel whose wavefronts hold 64 consecutive triangle ids.

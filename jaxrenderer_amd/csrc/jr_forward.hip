@@ -2,19 +2,22 @@
 //
 // Replaces the body of renderer/pipeline.py:render (vertex stage, per-primitive
 // setup, the scan over rows x vmap over pixels x vmap over primitives, chooser,
-// interpolate, fragment, mix and merge) with five kernel stages:
+// interpolate, fragment, mix and merge) with six kernel stages:
 //
-//   1. k_setup   one thread per triangle: clip-space corners (vertex stage
-//                positions), det / keep / LU inverse (PerPrimitive.create,
-//                renderer/pipeline.py:76-113), a conservative screen-tile range
-//                and a count of the tiles whose pixels it may cover.
-//   2. k_scan_*  exclusive scan of the per-tile counts (all scenes at once).
-//   3. k_bin     one thread per triangle writes its id into every tile list.
-//   4. k_raster  one wavefront-sized group per (scene, tile): stages the tile's
-//                triangle records in LDS, keeps the nearest (depth, id) key per
-//                pixel (the default primitive_chooser: argmin with first-index
-//                ties), then shades the winner and writes the merged buffers
-//                with stores along the contiguous height axis.
+//   1. k_clear   per-pixel keys, tile counters, per-scene sample tables and the
+//                scene of every 64-triangle wavefront.
+//   2. k_setup   per triangle: clip-space corners (vertex stage positions),
+//                det / keep / LU inverse (PerPrimitive.create,
+//                renderer/pipeline.py:76-113) and a conservative pixel range;
+//                small triangles resolved into the key buffer, the others
+//                counted into the tiles they may cover.
+//   3. k_wide    the tile counts of triangles spanning many tiles.
+//   4. k_scan_*  exclusive scan of the per-tile counts (all scenes at once).
+//   5. k_bin     triangle ids into the tile lists (k_wide for the wide ones).
+//   6. k_raster  one group per (scene, tile): stages the tile's triangle
+//                records in LDS, keeps the nearest (depth, id) key per pixel
+//                (the default primitive_chooser: argmin with first-index ties),
+//                then shades the winner and writes the merged buffers.
 //
 // Visibility is decided with exactly the reference's per-pixel arithmetic
 // (renderer/pipeline.py:170-279); binning only removes (triangle, tile) pairs
@@ -717,7 +720,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(
 // wavefront-local.
 __global__ __launch_bounds__(256) void k_bin(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
-    const float4* __restrict__ rec, const Rect* __restrict__ rects,
+    const Rect* __restrict__ rects,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
     int32_t* __restrict__ bins, const uint32_t* __restrict__ masks,
     const int32_t* __restrict__ wave_scene) {
@@ -1539,7 +1542,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   timer.mark(4);
   if (n_tris > 0) {
     hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rec, rects, offsets, cursor, bins, masks, wave_scene);
+                       scenes, f, rects, offsets, cursor, bins, masks, wave_scene);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
                        rects, wide, wide_count, counts, offsets, cursor, bins, samples, wbits);
   }
