@@ -1083,7 +1083,7 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
     }
     tile_sync<WAVE>();
     for (int m = 0; m < n; ++m) {
-      const int32_t t = lds_id[m];
+      const int32_t t = __builtin_amdgcn_readfirstlane(lds_id[m]);
       if (t < 0) continue;
       const float4 r0 = lds_rec[3 * m + 0], r1 = lds_rec[3 * m + 1],
                    r2 = lds_rec[3 * m + 2];
@@ -1109,6 +1109,7 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
         }
         if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
           const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
+          // a 64-bit (key, id) comparison instead is slower here
           if (key < best_key[q] || (key == best_key[q] && t < best_id[q])) {
             best_key[q] = key;
             best_id[q] = t;

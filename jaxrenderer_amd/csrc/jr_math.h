@@ -211,10 +211,12 @@ JR_DEV float modf_frac(float x) {
 // Total order key of a depth for the argmin: NaN first (jnp.argmin returns the
 // first NaN), -0 == +0, otherwise the IEEE order.
 JR_DEV uint32_t depth_key(float z) {
-  if (z != z) return 0u;
-  if (z == 0.0f) z = 0.0f;
-  uint32_t u = __float_as_uint(z);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  // z + 0 is +0 for z = -0 (round to nearest) and z itself otherwise; a
+  // negative value has all its bits flipped, any other gets the sign bit set
+  const float p = z + 0.0f;
+  const uint32_t u = __float_as_uint(p);
+  const uint32_t flip = (uint32_t)((int32_t)u >> 31) | 0x80000000u;
+  return (p != p) ? 0u : (u ^ flip);
 }
 
 constexpr uint32_t kInfKey = 0xFF800000u;  // depth_key(+inf)
