@@ -205,7 +205,8 @@ int32_t jr_version(void);
 const char* jr_last_error(void);
 
 /* sizeof of the ABI structs, for binding-side layout checks:
- * 0 jr_scene, 1 jr_inputs, 2 jr_outputs, 3 jr_frame, 4 jr_grads. */
+ * 0 jr_scene, 1 jr_inputs, 2 jr_outputs, 3 jr_frame, 4 jr_grads,
+ * 5 jr_pose_args. */
 int64_t jr_sizeof(int32_t which);
 
 /* Bytes of workspace jr_forward needs for this frame (bin_capacity included).
@@ -267,6 +268,44 @@ int32_t jr_backward(const jr_frame* frame, const jr_inputs* inputs,
                     const float* d_zbuffer, const int32_t* prim_id,
                     jr_grads* grads, void* workspace, int64_t workspace_bytes,
                     jr_stream_t stream);
+
+/* Object poses of a batch: the per-object world transform of merge_objects
+ * (renderer/model.py:447-555), for every object of every scene in one call,
+ * so that a batch whose objects only move is updated on the device.
+ * Object o owns model vertices [vert_begin[o], vert_begin[o + 1]) and normals
+ * [norm_begin[o], norm_begin[o + 1]); transforms[o] is its row-major 4x4 pose
+ * T and scaling[o] its local scaling.  Writes
+ *   positions[v] = to_cartesian(T @ [local_verts[v] * scaling[o], 1])
+ *     (Camera.apply_pos, renderer/geometry.py:321-346, sums left to right);
+ *   normal_matrices[o] = inv(T)^T (renderer/model.py:526, jnp.linalg.inv in
+ *     the canonical LU order of the triangle setup);
+ *   h = normal_matrices[o] @ [unit_norms[i], 0], normals[i] = h[:3] / s with s
+ *     the 2-norm of all the object's h[:3] together (Camera.apply_vec,
+ *     renderer/geometry.py:352-386; geometry.normalise, :39-50), its sum of
+ *     squares added in the order of numpy's float32 np.sum (8192-value
+ *     chunks added in turn, each summed pairwise).
+ * unit_norms are the model's normals already divided by their own
+ * whole-array norm (the first normalise of apply_vec, which no pose changes).
+ * vert_object[v] is the object of vertex v.  normals may be NULL (positions
+ * only).  An object may own at most JR_POSE_MAX_NORMALS normals. */
+#define JR_POSE_MAX_NORMALS 43690
+typedef struct jr_pose_args {
+  int32_t n_objects;
+  int32_t n_verts;
+  int32_t n_norms;
+  int32_t reserved;
+  const int32_t* vert_begin;  /* [n_objects + 1] */
+  const int32_t* norm_begin;  /* [n_objects + 1] */
+  const int32_t* vert_object; /* [n_verts] */
+  const float* local_verts;   /* [n_verts, 3] */
+  const float* unit_norms;    /* [n_norms, 3] */
+  const float* scaling;       /* [n_objects, 3] */
+  const float* transforms;    /* [n_objects, 16] */
+  float* normal_matrices;     /* [n_objects, 16], written */
+  float* positions;           /* [n_verts, 3], written */
+  float* normals;             /* [n_norms, 3], written; NULL: skipped */
+} jr_pose_args;
+int32_t jr_pose(const jr_pose_args* args, jr_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,28 @@ class JrGrads(ctypes.Structure):
     ]
 
 
+class JrPoseArgs(ctypes.Structure):
+    _fields_ = [
+        ("n_objects", c_i32),
+        ("n_verts", c_i32),
+        ("n_norms", c_i32),
+        ("reserved", c_i32),
+        ("vert_begin", c_vp),
+        ("norm_begin", c_vp),
+        ("vert_object", c_vp),
+        ("local_verts", c_vp),
+        ("unit_norms", c_vp),
+        ("scaling", c_vp),
+        ("transforms", c_vp),
+        ("normal_matrices", c_vp),
+        ("positions", c_vp),
+        ("normals", c_vp),
+    ]
+
+
+# largest normal count of one object jr_pose accepts (include/jrhip.h)
+POSE_MAX_NORMALS = 43690
+
 DSCENE_FLOATS = 69
 DSCENE = {
     "light_direction": 0,
@@ -156,7 +178,7 @@ DSCENE = {
     "shadow_strength": 66,
 }
 
-STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads)
+STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads, JrPoseArgs)
 
 
 class HipError(RuntimeError):
@@ -186,6 +208,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.jr_backward_workspace_size.argtypes = [P(JrFrame), c_i32]
     lib.jr_backward_workspace_size.restype = c_i64
     lib.jr_backward_last_error.restype = ctypes.c_char_p
+    lib.jr_pose.argtypes = [P(JrPoseArgs), c_vp]
+    lib.jr_pose.restype = c_i32
 
 
 def check_layout(lib: ctypes.CDLL) -> None:

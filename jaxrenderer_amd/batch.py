@@ -224,6 +224,11 @@ class BatchRenderer:
         main = self.passes["main"]
         if main.call.shadow_from is not None:
             main.inputs["shadow_maps"] = self.passes[main.call.shadow_from].zbuffer
+        for p in self.passes.values():
+            # one device copy of the world positions both passes read (and a
+            # pose update rewrites)
+            if p is not main and p.call.inputs.get("positions") is main.call.inputs["positions"]:
+                p.inputs["positions"] = main.inputs["positions"]
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self._side: Optional[torch.cuda.Stream] = None
 

@@ -1448,6 +1448,7 @@ int64_t jr_sizeof(int32_t which) {
     case 2: return sizeof(jr_outputs);
     case 3: return sizeof(jr_frame);
     case 4: return sizeof(jr_grads);
+    case 5: return sizeof(jr_pose_args);
     default: return -1;
   }
 }

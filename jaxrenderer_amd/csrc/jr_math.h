@@ -74,91 +74,98 @@ JR_DEV float det3(const float a[9]) {
   return d;
 }
 
-// jnp.linalg.inv (renderer/pipeline.py:105): LU with partial pivoting in the
-// operation order of LAPACK sgetrf2 (first maximal |pivot|, reciprocal scaling
-// when |pivot| >= FLT_MIN), then the permuted identity solved with the
-// reference-BLAS strsm loops (lower unit, then upper), skipping zero entries.
-JR_DEV void inv3(const float A[9], float X[9]) {
+// jnp.linalg.inv (renderer/pipeline.py:105; renderer/model.py:526 for the 4x4
+// object transforms): LU with partial pivoting in the operation order of
+// LAPACK sgetrf2 (first maximal |pivot|, reciprocal scaling when |pivot| >=
+// FLT_MIN), then the permuted identity solved with the reference-BLAS strsm
+// loops (lower unit, then upper), skipping zero entries.  A and X are N x N
+// row-major.
+template <int N>
+JR_DEV void inv_lu(const float* A, float* X) {
   // every index below is a compile-time constant (the pivot row is chosen by
   // conditional swaps with each candidate row), so the matrix stays in
   // registers
-  float a[9];
+  float a[N * N];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) a[i] = A[i];
-  int perm[3] = {0, 1, 2};
+  for (int i = 0; i < N * N; ++i) a[i] = A[i];
+  int perm[N];
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
+  for (int i = 0; i < N; ++i) perm[i] = i;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
     int p = j;
-    float amax = fabsf(a[3 * j + j]);
-    float apj = a[3 * j + j];
+    float amax = fabsf(a[N * j + j]);
+    float apj = a[N * j + j];
 #pragma unroll
-    for (int i = j + 1; i < 3; ++i) {
-      const float v = fabsf(a[3 * i + j]);
+    for (int i = j + 1; i < N; ++i) {
+      const float v = fabsf(a[N * i + j]);
       if (v > amax) {
         amax = v;
         p = i;
-        apj = a[3 * i + j];
+        apj = a[N * i + j];
       }
     }
     const bool nz = apj != 0.0f;
 #pragma unroll
-    for (int i = j + 1; i < 3; ++i) {
+    for (int i = j + 1; i < N; ++i) {
       const bool sw = nz && p == i;  // at most one candidate row swaps with row j
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float t = a[3 * i + k], u = a[3 * j + k];
-        a[3 * i + k] = sw ? u : t;
-        a[3 * j + k] = sw ? t : u;
+      for (int k = 0; k < N; ++k) {
+        const float t = a[N * i + k], u = a[N * j + k];
+        a[N * i + k] = sw ? u : t;
+        a[N * j + k] = sw ? t : u;
       }
       const int tp = perm[i], up = perm[j];
       perm[i] = sw ? up : tp;
       perm[j] = sw ? tp : up;
     }
     if (nz) {
-      const float piv = a[3 * j + j];
+      const float piv = a[N * j + j];
       if (fabsf(piv) >= 1.17549435e-38f) {
         const float r = 1.0f / piv;
 #pragma unroll
-        for (int i = j + 1; i < 3; ++i) a[3 * i + j] = a[3 * i + j] * r;
+        for (int i = j + 1; i < N; ++i) a[N * i + j] = a[N * i + j] * r;
       } else {
 #pragma unroll
-        for (int i = j + 1; i < 3; ++i) a[3 * i + j] = a[3 * i + j] / piv;
+        for (int i = j + 1; i < N; ++i) a[N * i + j] = a[N * i + j] / piv;
       }
     }
 #pragma unroll
-    for (int i = j + 1; i < 3; ++i)
+    for (int i = j + 1; i < N; ++i)
 #pragma unroll
-      for (int k = j + 1; k < 3; ++k)
-        a[3 * i + k] = a[3 * i + k] - a[3 * i + j] * a[3 * j + k];
+      for (int k = j + 1; k < N; ++k)
+        a[N * i + k] = a[N * i + k] - a[N * i + j] * a[N * j + k];
   }
-  float x[9];
+  float x[N * N];
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
+  for (int r = 0; r < N; ++r)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) x[3 * r + c] = (perm[r] == c) ? 1.0f : 0.0f;
+    for (int c = 0; c < N; ++c) x[N * r + c] = (perm[r] == c) ? 1.0f : 0.0f;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < N; ++c) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (x[3 * k + c] != 0.0f) {
+    for (int k = 0; k < N; ++k) {
+      if (x[N * k + c] != 0.0f) {
 #pragma unroll
-        for (int i = k + 1; i < 3; ++i)
-          x[3 * i + c] = x[3 * i + c] - x[3 * k + c] * a[3 * i + k];
+        for (int i = k + 1; i < N; ++i)
+          x[N * i + c] = x[N * i + c] - x[N * k + c] * a[N * i + k];
       }
     }
 #pragma unroll
-    for (int k = 2; k >= 0; --k) {
-      if (x[3 * k + c] != 0.0f) {
-        x[3 * k + c] = x[3 * k + c] / a[3 * k + k];
+    for (int k = N - 1; k >= 0; --k) {
+      if (x[N * k + c] != 0.0f) {
+        x[N * k + c] = x[N * k + c] / a[N * k + k];
 #pragma unroll
         for (int i = 0; i < k; ++i)
-          x[3 * i + c] = x[3 * i + c] - x[3 * k + c] * a[3 * i + k];
+          x[N * i + c] = x[N * i + c] - x[N * k + c] * a[N * i + k];
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < 9; ++i) X[i] = x[i];
+  for (int i = 0; i < N * N; ++i) X[i] = x[i];
 }
+
+JR_DEV void inv3(const float A[9], float X[9]) { inv_lu<3>(A, X); }
 
 // ---- scalar conversions with JAX semantics --------------------------------
 

@@ -19,7 +19,7 @@ Everything is drawn from numpy.random.Generator(PCG64(seed)).
 from __future__ import annotations
 
 import math
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -123,11 +123,17 @@ def ring_camera(i: int, n: int, width: int, height: int, radius: float = 8.0,
 
 
 def config3(n_scenes: int = 1024, seed: int = 0, width: int = 84, height: int = 84,
-            shadow_size: int = 256) -> Tuple[List[BatchScene], ShadowMapSpec]:
+            shadow_size: int = 256, objects: Optional[list] = None
+            ) -> Tuple[List[BatchScene], ShadowMapSpec]:
+    """`objects`: a list that receives each scene's ModelObject list (for
+    a ScenePoser)."""
     rng = _rng(seed)
     scenes = []
     for i in range(n_scenes):
-        model = merge_objects(brax_like_objects(rng))
+        objs = brax_like_objects(rng)
+        if objects is not None:
+            objects.append(objs)
+        model = merge_objects(objs)
         scenes.append(BatchScene(model=model, camera=ring_camera(i, n_scenes, width, height),
                                  light=NOTEBOOK_LIGHT))
     return scenes, ShadowMapSpec(width=shadow_size, height=shadow_size)
