@@ -56,6 +56,7 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-scene", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
+    ap.add_argument("--no-pose", action="store_true", help="skip timing the pose update")
     ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
                     help="c3 (default): 1024 Brax-like scenes at 84x84; c4: 64-view "
                          "differentiation of a 5000-triangle sphere at 256x256; c5: 512 "
@@ -90,7 +91,9 @@ def main() -> None:
     W = H = 84
     B = args.scenes
     t0 = time.time()
-    scenes, smspec = workloads.config3(n_scenes=B, seed=args.seed, width=W, height=H)
+    objects: list = []
+    scenes, smspec = workloads.config3(n_scenes=B, seed=args.seed, width=W, height=H,
+                                       objects=objects)
     tris = workloads.triangle_counts(scenes)
     calls = batch.plan_batch(scenes, W, H, smspec)
     log(f"generated {B} scenes ({tris.sum()} triangles, {tris.min()}..{tris.max()} per scene) "
@@ -150,6 +153,41 @@ def main() -> None:
     stages.update({f"main.{n}": float(v) for n, v in zip(names, main_ms)})
     dom = max(stages, key=stages.get)
     log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+
+    # moving every object of the batch on the device (jr_pose): the world
+    # positions and normals of all scenes for a new [n_objects, 4, 4] stack of
+    # poses, beside the host merge_objects it replaces (timed on a prefix)
+    pose = None
+    if not args.no_pose:
+        from jaxrenderer_amd.model import merge_objects
+        from jaxrenderer_amd.pose import ScenePoser
+
+        poser = ScenePoser(objects)
+        T = torch.from_numpy(np.ascontiguousarray(poser.initial_transforms)).cuda()
+        for _ in range(3):
+            poser.update(br, T)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            poser.update(br, T)
+        e1.record()
+        torch.cuda.synchronize()
+        pose_ms = e0.elapsed_time(e1) / args.steps
+        n_host = min(B, 32)
+        t0 = time.perf_counter()
+        for s in objects[:n_host]:
+            merge_objects(s)
+        host_ms = (time.perf_counter() - t0) * 1e3 * B / n_host
+        out = br.render()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(out["canvas"], canvas_b))
+        pose = {"objects": poser.n_objects, "vertices": poser.n_verts, "normals": poser.n_norms,
+                "device_ms": round(pose_ms, 4), "host_merge_ms": round(host_ms, 1),
+                "render_unchanged": same}
+        log(f"pose update of {poser.n_objects} objects ({poser.n_verts} vertices): "
+            f"{pose_ms:.3f} ms on the device vs {host_ms:.0f} ms of host merge_objects "
+            f"(extrapolated from {n_host} scenes); render unchanged: {same}")
 
 
     # forward + backward: L2 image loss against a fixed synthetic target,
@@ -326,6 +364,7 @@ def main() -> None:
         "fwd_bwd_over_fwd": None if fwd_bwd_fps is None else round(fwd_bwd_fps / fps, 3),
         "fwd_bwd_grads_finite": grads_finite,
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "pose_update": pose,
         "binned_pairs": pairs,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -287,13 +287,15 @@ int32_t jr_backward(const jr_frame* frame, const jr_inputs* inputs,
  * unit_norms are the model's normals already divided by their own
  * whole-array norm (the first normalise of apply_vec, which no pose changes).
  * vert_object[v] is the object of vertex v.  normals may be NULL (positions
- * only).  An object may own at most JR_POSE_MAX_NORMALS normals. */
+ * only).  max_object_norms is the largest normal count of any object (at most
+ * JR_POSE_MAX_NORMALS; it sizes the normal kernel's shared memory): an object
+ * with more normals is left unwritten. */
 #define JR_POSE_MAX_NORMALS 43690
 typedef struct jr_pose_args {
   int32_t n_objects;
   int32_t n_verts;
   int32_t n_norms;
-  int32_t reserved;
+  int32_t max_object_norms;
   const int32_t* vert_begin;  /* [n_objects + 1] */
   const int32_t* norm_begin;  /* [n_objects + 1] */
   const int32_t* vert_object; /* [n_verts] */

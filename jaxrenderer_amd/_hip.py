@@ -147,7 +147,7 @@ class JrPoseArgs(ctypes.Structure):
         ("n_objects", c_i32),
         ("n_verts", c_i32),
         ("n_norms", c_i32),
-        ("reserved", c_i32),
+        ("max_object_norms", c_i32),
         ("vert_begin", c_vp),
         ("norm_begin", c_vp),
         ("vert_object", c_vp),

@@ -34,10 +34,14 @@ namespace {
 constexpr int kPoseThreads = 256;
 constexpr int kLeafLen = 128;  // numpy's pairwise block size
 constexpr int kChunk = 8192;   // numpy's reduction buffer, in values
-// leaves of one object's pairwise trees: the halving keeps every leaf of a
-// chunk of more than 128 values at 64 or more, and a shorter chunk is one leaf
-constexpr int kMaxLeaves =
-    (3 * JR_POSE_MAX_NORMALS + 63) / 64 + (3 * JR_POSE_MAX_NORMALS + kChunk - 1) / kChunk;
+
+// Leaves of the pairwise trees of an object with at most m normals: the
+// halving keeps every leaf of a chunk of more than 128 values at 64 or more,
+// and a shorter chunk is one leaf.
+int32_t pose_leaf_cap(int32_t m) {
+  const int64_t n = 3 * (int64_t)m;
+  return (int32_t)(n / 64 + (n + kChunk - 1) / kChunk);
+}
 
 __global__ __launch_bounds__(kPoseThreads) void k_pose_matrices(int32_t n_objects,
                                                                 const float* __restrict__ T,
@@ -83,42 +87,6 @@ JR_DEV float pose_h(const float* __restrict__ unit, const float* M, int64_t firs
   return s;
 }
 
-// numpy's pairwise sum of a block of at most 128 squares: fewer than 8 values
-// are added to 0 one by one; otherwise eight partial sums over strides of 8,
-// combined as ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)), then the
-// values past the last multiple of 8 added one by one.
-JR_DEV float pose_leaf(const float* __restrict__ unit, const float* M, int64_t first, int32_t s,
-                       int32_t n) {
-  if (n < 8) {
-    float res = 0.0f;
-    for (int32_t i = 0; i < n; ++i) {
-      const float h = pose_h(unit, M, first, s + i);
-      res = res + h * h;
-    }
-    return res;
-  }
-  float r[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const float h = pose_h(unit, M, first, s + j);
-    r[j] = h * h;
-  }
-  int32_t i = 8;
-  for (; i < n - (n % 8); i += 8) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float h = pose_h(unit, M, first, s + i + j);
-      r[j] = r[j] + h * h;
-    }
-  }
-  float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-  for (; i < n; ++i) {
-    const float h = pose_h(unit, M, first, s + i);
-    res = res + h * h;
-  }
-  return res;
-}
-
 // The halving of numpy's pairwise sum: a range of more than 128 values splits
 // at n2 = n / 2 rounded down to a multiple of 8.
 JR_DEV int32_t pose_split(int32_t n) {
@@ -126,29 +94,51 @@ JR_DEV int32_t pose_split(int32_t n) {
   return n2 - n2 % 8;
 }
 
+// Objects of at most this many normal components keep their transformed
+// normals in LDS between the passes; larger ones recompute them.
+constexpr int kStageMax = 8192;
+
+// Shared memory: cap leaf starts, cap leaf lengths, cap leaf sums, then (when
+// stage) 3 * max_norms transformed components.
+//
+// numpy's pairwise sum of a leaf (at most 128 squares): fewer than 8 values
+// are added to 0 one by one; otherwise eight partial sums over strides of 8
+// (one lane each here), combined as ((r0 + r1) + (r2 + r3)) + ((r4 + r5) +
+// (r6 + r7)), then the values past the last multiple of 8 added one by one.
 __global__ __launch_bounds__(64) void k_pose_normals(const int32_t* __restrict__ norm_begin,
                                                      const float* __restrict__ unit,
                                                      const float* __restrict__ nm,
-                                                     float* __restrict__ out) {
-  __shared__ int32_t leaf_s[kMaxLeaves], leaf_n[kMaxLeaves];
-  __shared__ float leaf_sum[kMaxLeaves];
+                                                     float* __restrict__ out,
+                                                     int32_t max_norms, int32_t cap, int32_t stage) {
+  extern __shared__ int32_t pose_lds[];
+  int32_t* leaf_s = pose_lds;
+  int32_t* leaf_n = pose_lds + cap;
+  float* leaf_sum = reinterpret_cast<float*>(pose_lds + 2 * cap);
+  float* hs = reinterpret_cast<float*>(pose_lds + 3 * cap);
+  __shared__ float part[64];
   __shared__ int32_t n_leaves;
   __shared__ float norm;
+  // lane 0's walk stacks (in LDS: a private array indexed at run time would
+  // live in scratch memory, and every step of the walks depends on the last)
+  __shared__ int32_t st_s[32], st_n[32], st_has[32];
+  __shared__ float st_left[32];
   const int32_t o = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t first = norm_begin[o];
   const int32_t count = norm_begin[o + 1] - norm_begin[o];
-  // an object beyond the limit is left unwritten (its leaves would not fit)
-  if (count < 0 || count > JR_POSE_MAX_NORMALS) return;
+  // an object beyond the stated maximum is left unwritten (its leaves would
+  // not fit)
+  if (count < 0 || count > max_norms) return;
   const int32_t n = 3 * count;
   float M[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) M[i] = nm[16 * (int64_t)o + i];
+  if (stage)
+    for (int32_t k = lane; k < n; k += 64) hs[k] = pose_h(unit, M, first, k);
 
   // the leaves of every chunk's pairwise tree, left to right (a depth-first
   // walk that visits the left half of every split first)
   if (lane == 0) {
-    int32_t st_s[32], st_n[32];
     int32_t k = 0;
     for (int32_t cs = 0; cs < n; cs += kChunk) {
       st_s[0] = cs;
@@ -175,8 +165,42 @@ __global__ __launch_bounds__(64) void k_pose_normals(const int32_t* __restrict__
   }
   __syncthreads();
   const int32_t L = n_leaves;
-  for (int32_t k = lane; k < L; k += 64) leaf_sum[k] = pose_leaf(unit, M, first, leaf_s[k], leaf_n[k]);
-  __syncthreads();
+  // eight leaves at a time: partial sum j of leaf base + q on lane 8q + j,
+  // then lane q combines leaf base + q's eight
+  for (int32_t base = 0; base < L; base += 8) {
+    {
+      const int32_t k = base + (lane >> 3), j = lane & 7;
+      float r = 0.0f;
+      if (k < L && leaf_n[k] >= 8) {
+        const int32_t s = leaf_s[k], len = leaf_n[k];
+        float h = stage ? hs[s + j] : pose_h(unit, M, first, s + j);
+        r = h * h;
+        for (int32_t i = 8; i < len - (len % 8); i += 8) {
+          h = stage ? hs[s + i + j] : pose_h(unit, M, first, s + i + j);
+          r = r + h * h;
+        }
+      }
+      part[lane] = r;
+    }
+    __syncthreads();
+    const int32_t k = base + lane;
+    if (lane < 8 && k < L) {
+      const int32_t s = leaf_s[k], len = leaf_n[k];
+      const float* r = part + 8 * lane;
+      float res = 0.0f;
+      int32_t i = 0;
+      if (len >= 8) {
+        res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        i = len - (len % 8);
+      }
+      for (; i < len; ++i) {
+        const float h = stage ? hs[s + i] : pose_h(unit, M, first, s + i);
+        res = res + h * h;
+      }
+      leaf_sum[k] = res;
+    }
+    __syncthreads();
+  }
 
   // combine each chunk's leaves along its tree (post-order: each split adds
   // its left half's sum to its right half's), and the chunks' sums in turn
@@ -184,16 +208,13 @@ __global__ __launch_bounds__(64) void k_pose_normals(const int32_t* __restrict__
     float total = 0.0f;
     int32_t next = 0;
     for (int32_t cs = 0; cs < n; cs += kChunk) {
-      int32_t st_n[32];
-      float st_left[32];
-      bool st_has[32];
       int32_t d = 0, len = min(kChunk, n - cs);
       float v;
       for (;;) {
         while (len > kLeafLen) {
           const int32_t n2 = pose_split(len);
           st_n[d] = len - n2;  // the right half still to do
-          st_has[d] = false;
+          st_has[d] = 0;
           ++d;
           len = n2;
         }
@@ -204,7 +225,7 @@ __global__ __launch_bounds__(64) void k_pose_normals(const int32_t* __restrict__
         }
         if (d == 0) break;
         st_left[d - 1] = v;  // a left half done: its right half is next
-        st_has[d - 1] = true;
+        st_has[d - 1] = 1;
         len = st_n[d - 1];
       }
       total = total + v;
@@ -212,8 +233,9 @@ __global__ __launch_bounds__(64) void k_pose_normals(const int32_t* __restrict__
     norm = sqrtf(total);
   }
   __syncthreads();
-  const float s = norm;
-  for (int32_t k = lane; k < n; k += 64) out[3 * first + k] = pose_h(unit, M, first, k) / s;
+  const float sn = norm;
+  for (int32_t k = lane; k < n; k += 64)
+    out[3 * first + k] = (stage ? hs[k] : pose_h(unit, M, first, k)) / sn;
 }
 
 }  // namespace
@@ -234,6 +256,9 @@ int32_t jr_pose(const jr_pose_args* a, jr_stream_t stream) {
     return fail(JR_E_INVALID, "jr_pose: vertices without vert_object, local_verts or positions");
   if (a->normals && (!a->norm_begin || (a->n_norms > 0 && !a->unit_norms)))
     return fail(JR_E_INVALID, "jr_pose: normals without norm_begin or unit_norms");
+  if (a->normals && (a->max_object_norms < 0 || a->max_object_norms > JR_POSE_MAX_NORMALS))
+    return fail(JR_E_INVALID, "jr_pose: max_object_norms must be in [0, " +
+                                  std::to_string(JR_POSE_MAX_NORMALS) + "]");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const unsigned g_obj = (unsigned)((a->n_objects + kPoseThreads - 1) / kPoseThreads);
   hipLaunchKernelGGL(k_pose_matrices, dim3(g_obj), dim3(kPoseThreads), 0, st, a->n_objects,
@@ -243,9 +268,14 @@ int32_t jr_pose(const jr_pose_args* a, jr_stream_t stream) {
     hipLaunchKernelGGL(k_pose_positions, dim3(g_v), dim3(kPoseThreads), 0, st, a->n_verts,
                        a->vert_object, a->local_verts, a->scaling, a->transforms, a->positions);
   }
-  if (a->normals)
-    hipLaunchKernelGGL(k_pose_normals, dim3((unsigned)a->n_objects), dim3(64), 0, st,
-                       a->norm_begin, a->unit_norms, a->normal_matrices, a->normals);
+  if (a->normals) {
+    const int32_t cap = pose_leaf_cap(a->max_object_norms);
+    const int32_t stage = 3 * a->max_object_norms <= kStageMax ? 1 : 0;
+    const size_t lds = (size_t)cap * 12 + (stage ? (size_t)3 * a->max_object_norms * 4 : 0);
+    hipLaunchKernelGGL(k_pose_normals, dim3((unsigned)a->n_objects), dim3(64), lds, st,
+                       a->norm_begin, a->unit_norms, a->normal_matrices, a->normals,
+                       a->max_object_norms, cap, stage);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(JR_E_HIP, std::string("jr_pose: ") + hipGetErrorString(e));
   return JR_OK;

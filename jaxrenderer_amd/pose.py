@@ -62,6 +62,7 @@ class ScenePoser:
         self.n_objects = len(vcount)
         self.n_verts = int(sum(vcount))
         self.n_norms = int(sum(ncount))
+        self.max_object_norms = int(max(ncount, default=0))
         vb = np.zeros(self.n_objects + 1, np.int64)
         vb[1:] = np.cumsum(vcount)
         nb = np.zeros(self.n_objects + 1, np.int64)
@@ -95,6 +96,7 @@ class ScenePoser:
             self._check(normals, self.n_norms, "normals")
         a = _hip.JrPoseArgs()
         a.n_objects, a.n_verts, a.n_norms = self.n_objects, self.n_verts, self.n_norms
+        a.max_object_norms = self.max_object_norms
         a.vert_begin = self.vert_begin.data_ptr()
         a.norm_begin = self.norm_begin.data_ptr()
         a.vert_object = self.vert_object.data_ptr()

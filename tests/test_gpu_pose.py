@@ -49,7 +49,7 @@ def moved(objects, T):
     return out
 
 
-def scene_objects():
+def scene_objects(big=True):
     objects = []
     workloads.config3(n_scenes=5, seed=31, objects=objects)
     rng = np.random.default_rng(3)
@@ -57,8 +57,9 @@ def scene_objects():
     spec = np.full((8, 8), 2.0, f32)
     # a dense sphere (a deep pairwise tree: ~14k normal components), a scaled
     # box, and a one-triangle object with a single normal
-    big = workloads.mesh_model(*workloads.uv_sphere(48, 96, 0.8), tex, spec)
-    objects[0].append(ModelObject(model=big))
+    if big:
+        dense = workloads.mesh_model(*workloads.uv_sphere(48, 96, 0.8), tex, spec)
+        objects[0].append(ModelObject(model=dense))
     objects[1].append(ModelObject(model=objects[1][1].model,
                                   local_scaling=np.array([1.5, 0.5, 2.0], f32)))
     one = Model(verts=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], f32),
@@ -69,10 +70,13 @@ def scene_objects():
     return objects
 
 
-def test_pose_matches_merge_objects():
-    objects = scene_objects()
+@pytest.mark.parametrize("big", [False, True], ids=["staged", "recomputed"])
+def test_pose_matches_merge_objects(big):
+    """Objects of at most 2730 normals keep their transformed normals in LDS;
+    with a larger one in the batch every object recomputes them."""
+    objects = scene_objects(big)
     poser = ScenePoser(objects)
-    assert max(len(o.model.norms) for s in objects for o in s) > 4000
+    assert (poser.max_object_norms > 4000) == big
     dev = poser.device
     for seed in (0, 1):
         T = poses(poser.n_objects, np.random.default_rng(seed))
