@@ -325,11 +325,11 @@ JR_DEV int32_t dpp_max_step(int32_t v) {
 // inclusive prefix maximum over the lanes (DPP: row shifts 1, 2, 4, 8, then
 // the row broadcasts 15 and 31) carries each owner forward.  Call with every
 // lane of the wavefront active.
-JR_DEV int32_t wave_owner(int32_t first, int32_t count, int32_t base, int32_t* mark,
+JR_DEV int32_t wave_owner(int32_t first, int32_t count, int32_t base, int8_t* mark,
                           int lane) {
   mark[lane] = -1;
   tile_sync<true>();
-  if (count > 0 && first > base && first < base + 64) mark[first - base] = lane;
+  if (count > 0 && first > base && first < base + 64) mark[first - base] = (int8_t)lane;
   tile_sync<true>();
   int32_t v = mark[lane];
   const uint64_t le = __ballot(first <= base);
@@ -380,7 +380,10 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene) {
   __shared__ SetupStage stage;
   __shared__ int32_t wave_live[kSetupThreads / 64];
-  __shared__ int32_t mark[kSetupThreads];  // wave_owner scratch, 64 per wavefront
+  // wave_owner scratch, 64 per wavefront, in bytes: with 4-byte entries the
+  // workgroup's LDS would pass 20 KiB, an eighth of the CU's 160 KiB, and 7
+  // workgroups would fit on a CU instead of 8
+  __shared__ int8_t mark[kSetupThreads];
   SmallStage& sm = stage.sm;
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
@@ -728,7 +731,7 @@ __global__ __launch_bounds__(256) void k_bin(
   __shared__ int32_t own_t[256];
   __shared__ uint32_t own_m[256];
   __shared__ int32_t own_off[256];  // exclusive prefix sum of the bit counts
-  __shared__ int32_t mark[256];     // wave_owner scratch, 64 per wavefront
+  __shared__ int8_t mark[256];      // wave_owner scratch, 64 per wavefront
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(t & ~63) >= in.n_tris) return;
   const int lane = threadIdx.x & 63;
