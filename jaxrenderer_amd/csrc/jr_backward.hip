@@ -822,9 +822,12 @@ JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
 
 // One thread per triangle: the touched ones (flag = scene + 1) are packed to
 // the front of the workgroup, read their accumulators and clear them with the
-// flag for the next call; wavefronts left without one return at once.
+// flag for the next call; wavefronts left without one return at once.  At
+// most 80 VGPRs (6 wavefronts per SIMD instead of 5): the kernel waits on its
+// gathers most of the time, and more wavefronts hide more of that.
 template <int SHADER>
-__global__ __launch_bounds__(256) void k_bwd_tris(BwdArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+void k_bwd_tris(BwdArgs a) {
   __shared__ int2 items[256];
   __shared__ int32_t wave_n[4];
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
