@@ -288,8 +288,9 @@ int32_t jr_backward(const jr_frame* frame, const jr_inputs* inputs,
  * whole-array norm (the first normalise of apply_vec, which no pose changes).
  * vert_object[v] is the object of vertex v.  normals may be NULL (positions
  * only).  max_object_norms is the largest normal count of any object (at most
- * JR_POSE_MAX_NORMALS; it sizes the normal kernel's shared memory): an object
- * with more normals is left unwritten. */
+ * JR_POSE_MAX_NORMALS, whose pairwise-sum tables take 24 KiB of shared
+ * memory; it sizes the normal kernel's shared memory): an object with more
+ * normals is left unwritten. */
 #define JR_POSE_MAX_NORMALS 43690
 typedef struct jr_pose_args {
   int32_t n_objects;

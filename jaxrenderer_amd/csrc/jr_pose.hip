@@ -26,8 +26,7 @@
 
 namespace jr {
 
-extern thread_local std::string g_err;
-int32_t fail(int32_t code, const std::string& msg);
+int32_t fail(int32_t code, const std::string& msg);  // jr_forward.hip: sets jr_last_error()
 
 namespace {
 
