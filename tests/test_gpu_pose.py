@@ -137,3 +137,39 @@ def test_pose_update_renders_like_a_replanned_batch():
             got = br._outputs()
             for k in ("zbuffer", "canvas", "shadow_map"):
                 assert torch.equal(got[k], want[k]), (seed, k)
+
+
+def test_limits_are_checked_before_any_launch():
+    """An object above the normal limit is refused by ScenePoser, and a
+    max_object_norms outside [0, JR_POSE_MAX_NORMALS] by jr_pose itself."""
+    import ctypes
+
+    from jaxrenderer_amd import _hip
+
+    n = _hip.POSE_MAX_NORMALS + 1
+    huge = Model(verts=np.zeros((3, 3), f32), norms=np.ones((n, 3), f32),
+                 uvs=np.zeros((3, 2), f32), faces=np.array([[0, 1, 2]], np.int32),
+                 faces_norm=np.zeros((1, 3), np.int32), faces_uv=np.array([[0, 1, 2]], np.int32),
+                 diffuse_map=np.zeros((2, 2, 3), f32), specular_map=np.zeros((2, 2), f32))
+    with pytest.raises(ValueError):
+        ScenePoser([[ModelObject(model=huge)]])
+
+    poser = ScenePoser(scene_objects(False)[:1])
+    dev = poser.device
+    T = torch.from_numpy(poser.initial_transforms.reshape(-1, 16).copy()).to(dev)
+    pos = torch.empty((poser.n_verts, 3), dtype=torch.float32, device=dev)
+    nrm = torch.empty((poser.n_norms, 3), dtype=torch.float32, device=dev)
+    a = _hip.JrPoseArgs()
+    a.n_objects, a.n_verts, a.n_norms = poser.n_objects, poser.n_verts, poser.n_norms
+    a.max_object_norms = _hip.POSE_MAX_NORMALS + 1
+    for name in ("vert_begin", "norm_begin", "vert_object", "local_verts", "unit_norms",
+                 "scaling", "normal_matrices"):
+        setattr(a, name, getattr(poser, name).data_ptr())
+    a.transforms, a.positions, a.normals = T.data_ptr(), pos.data_ptr(), nrm.data_ptr()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    lib = _hip.lib()
+    assert lib.jr_pose(ctypes.byref(a), stream) == -1
+    assert b"max_object_norms" in lib.jr_last_error()
+    a.max_object_norms = poser.max_object_norms
+    assert lib.jr_pose(ctypes.byref(a), stream) == 0
+    torch.cuda.synchronize()
