@@ -297,7 +297,8 @@ typedef struct jr_pose_args {
   int32_t n_verts;
   int32_t n_norms;
   int32_t max_object_norms;
-  const int32_t* vert_begin;  /* [n_objects + 1] */
+  const int32_t* vert_begin;  /* [n_objects + 1]; optional, not read: vert_object
+                                 carries the same ownership per vertex */
   const int32_t* norm_begin;  /* [n_objects + 1] */
   const int32_t* vert_object; /* [n_verts] */
   const float* local_verts;   /* [n_verts, 3] */

@@ -107,6 +107,7 @@ def render_custom(camera, shader: type, buffers: Buffers, face_indices, extra,
     cam = _to_tensors(camera, dev)
     faces = _t(face_indices, dev, torch.int64)
     V = int(ex[0].shape[0])
+    _plan.check_index(faces, V, "face_indices")
     inst = torch.zeros((), dtype=torch.int64, device=dev)
     per_vertex, varyings = vmap(lambda vid: shader.vertex(vid, inst, cam, ex))(
         torch.arange(V, device=dev))

@@ -21,6 +21,36 @@ _FLOAT_INPUTS = {"positions", "normals", "uvs", "colours", "textures", "specular
                  "normal_maps", "shadow_maps", "clip"}
 
 
+def check_index(idx, n: int, name: str) -> None:
+    """Raises ValueError unless every entry of the index array `idx` lies in
+    [0, n).  The kernels gather attr[idx] without a clamp, so an index out of
+    range would read past the array (or another scene's rows); the reference's
+    Model.value_checks (renderer/model.py:62-78) rejects the same inputs.
+    Device tensors are reduced on the device (one synchronisation)."""
+    if idx is None:
+        return
+    if isinstance(idx, torch.Tensor):
+        if idx.numel() == 0:
+            return
+        lo, hi = int(idx.min()), int(idx.max())
+    else:
+        a = np.asarray(idx)
+        if a.size == 0:
+            return
+        lo, hi = int(a.min()), int(a.max())
+    if lo < 0 or hi >= n:
+        raise ValueError(f"{name} out of bound, expected [0, {n}), got [{lo}, {hi}]")
+
+
+def check_rows(arr, n: int, name: str) -> None:
+    """Raises ValueError when the per-slot array `arr` has fewer than n rows."""
+    if arr is None:
+        return
+    rows = int(arr.shape[0]) if getattr(arr, "ndim", 1) > 0 else 0
+    if rows < n:
+        raise ValueError(f"{name} has {rows} rows; the faces index {n} vertex slots")
+
+
 @dataclass
 class RasterCall:
     shader: int

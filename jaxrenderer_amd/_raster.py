@@ -151,7 +151,10 @@ def workspace_for(stream: torch.cuda.Stream, nbytes: int, dev: torch.device,
     with _ws_lock:
         ws = _workspaces.setdefault(key, _Workspace())
         if ws.buf is None or ws.buf.numel() < nbytes:
-            ws.buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
+            # the backward's is zeroed once (its header then names the state
+            # it holds); recycled memory must not pass for a cleared workspace
+            alloc = torch.zeros if kind == "backward" else torch.empty
+            ws.buf = alloc(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
         return ws.buf
 
 

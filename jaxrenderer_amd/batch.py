@@ -57,6 +57,8 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
     pos_index, nrm_index, uv_index, slot_tex, shadow_faces = [], [], [], [], []
     tex, spec, tshape = [], [], []
     main_sc, shadow_sc = [], []
+    scene_verts: List[int] = []
+    scene_norms: List[int] = []
     voff = noff = uoff = toff = tex_off = spec_off = ts_off = 0
     if shadow is not None:
         svp = Camera.viewport_matrix(np.zeros(2, np.int32),
@@ -67,6 +69,13 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
         F = int(faces.shape[0])
         fuv = np.asarray(m.faces_uv, np.int32).reshape(-1)
         ti = np.asarray(m.texture_index, np.int32)
+        # indices become global below: one out of range would read another
+        # scene's rows (or past the arrays) on the device
+        _plan.check_index(faces, len(m.verts), "faces")
+        _plan.check_index(m.faces_norm, len(m.norms), "faces_norm")
+        _plan.check_index(fuv, len(m.uvs), "faces_uv")
+        if np.asarray(m.faces_norm).shape != faces.shape or fuv.size != faces.size:
+            raise ValueError("faces, faces_norm and faces_uv must all be [F, 3]")
         pos.append(asf32(m.verts))
         nrm.append(asf32(m.norms))
         uvs.append(asf32(m.uvs))
@@ -112,6 +121,8 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
                                                   viewport=scam.viewport, tri_begin=toff,
                                                   tri_end=toff + F))
         main_sc.append(_raster.scene_struct(**sc))
+        scene_verts.append(len(m.verts))
+        scene_norms.append(len(m.norms))
         voff += len(m.verts)
         noff += len(m.norms)
         uoff += len(m.uvs)
@@ -151,7 +162,8 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
         zbuffer_default=float(zbuffer_default),
         colour_default=tuple(float(c) for c in np.asarray(colour_default, f32).reshape(-1)[:3]),
         shadow_from="shadow" if shadow is not None else None, tag="main",
-        extras={"identity_faces": True}))
+        extras={"identity_faces": True, "scene_verts": scene_verts,
+                "scene_norms": scene_norms}))
     return calls
 
 
@@ -335,7 +347,9 @@ class BatchRenderer:
         need = _raster.backward_workspace_size(width=c.width, height=c.height,
                                                n_scenes=c.n_scenes, shader=c.shader,
                                                n_tris=c.n_tris)
-        self.bwd_workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+        # zeroed once: recycled memory whose header happened to match a cleared
+        # workspace would otherwise skip the first full clear
+        self.bwd_workspace = torch.zeros(need, dtype=torch.uint8, device=dev)
         self.graph = None
         return self.grads
 

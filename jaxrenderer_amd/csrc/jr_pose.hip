@@ -249,8 +249,8 @@ int32_t jr_pose(const jr_pose_args* a, jr_stream_t stream) {
   if (a->n_objects < 0 || a->n_verts < 0 || a->n_norms < 0)
     return fail(JR_E_INVALID, "jr_pose: negative count");
   if (a->n_objects == 0) return JR_OK;
-  if (!a->vert_begin || !a->transforms || !a->normal_matrices || !a->scaling)
-    return fail(JR_E_INVALID, "jr_pose: vert_begin, scaling, transforms and normal_matrices are required");
+  if (!a->transforms || !a->normal_matrices || !a->scaling)
+    return fail(JR_E_INVALID, "jr_pose: scaling, transforms and normal_matrices are required");
   if (a->n_verts > 0 && (!a->vert_object || !a->local_verts || !a->positions))
     return fail(JR_E_INVALID, "jr_pose: vertices without vert_object, local_verts or positions");
   if (a->normals && (!a->norm_begin || (a->n_norms > 0 && !a->unit_norms)))

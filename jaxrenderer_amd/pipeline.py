@@ -57,6 +57,15 @@ def build_call(camera: Camera, shader: type, face_indices, extra, width: int, he
     faces = _arr(face_indices)
     n_tris = int(faces.shape[0])
     n_slots = int(extra[0].shape[0])
+    # every slot a face names must exist in every per-vertex array
+    _plan.check_index(faces, n_slots, "face_indices")
+    for name in ("normal", "uv", "colour", "texture_index", "id_to_face"):
+        if name in getattr(extra, "_fields", ()):
+            _plan.check_rows(getattr(extra, name), n_slots, f"extra.{name}")
+    if sid == _hip.SHADER_PHONG_DARBOUX:
+        fi = _arr(extra.faces_indices)
+        _plan.check_index(fi, n_slots, "extra.faces_indices")
+        _plan.check_index(_arr(extra.id_to_face), int(fi.shape[0]), "extra.id_to_face")
     inputs = {"positions": _arr(extra.position), "faces": faces}
     sc = dict(
         world_to_clip=camera.world_to_clip,

@@ -44,7 +44,13 @@ class ScenePoser:
         scaling: List[np.ndarray] = []
         transforms: List[np.ndarray] = []
         self.object_begin = [0]
+        self.scene_verts: List[int] = []
+        self.scene_norms: List[int] = []
         for scene in objects:
+            self.scene_verts.append(sum(len(np.asarray(o.model.verts).reshape(-1, 3))
+                                        for o in scene))
+            self.scene_norms.append(sum(len(np.asarray(o.model.norms).reshape(-1, 3))
+                                        for o in scene))
             for o in scene:
                 v = np.asarray(o.model.verts, f32).reshape(-1, 3)
                 n = np.asarray(o.model.norms, f32).reshape(-1, 3)
@@ -118,6 +124,11 @@ class ScenePoser:
         shadow pass, and the main pass's normals are rewritten in place, so a
         captured graph replays the new poses."""
         main = renderer.passes["main"]
+        ex = main.call.extras
+        if (ex.get("scene_verts") != self.scene_verts
+                or ex.get("scene_norms") != self.scene_norms):
+            raise ValueError("the renderer was planned from scenes whose per-scene vertex or "
+                             "normal counts differ from this poser's objects")
         self.pose(transforms, main.inputs["positions"], main.inputs.get("normals"))
 
     def _check(self, t: torch.Tensor, n: int, name: str) -> None:

@@ -55,6 +55,8 @@ class Shadow(NamedTuple):
             sm = torch.as_tensor(np.asarray(sm))
         pos = torch.as_tensor(position, device=sm.device)
         r = torch.sign(pos[..., :2]) * torch.floor(torch.abs(pos[..., :2]) + 0.5)
+        # XLA's float -> int32 convert: NaN -> 0, saturating at the int32 range
+        r = torch.nan_to_num(r, nan=0.0).clamp(-2147483648.0, 2147483647.0)
         ix = r[..., 0].to(torch.int64)
         iy = r[..., 1].to(torch.int64)
         W, H = sm.shape[0], sm.shape[1]

@@ -1,4 +1,5 @@
-"""Loads the mesh constant tables (see extract_tables.py for their origin)."""
+"""Loads the mesh constant tables (written by tests/golden/extract_tables.py
+from the reference's renderer/shapes/cube.py and capsule.py)."""
 import os
 
 import numpy as np

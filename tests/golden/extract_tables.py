@@ -4,10 +4,10 @@ The cube and capsule meshes of renderer/shapes/cube.py:16-140 and
 renderer/shapes/capsule.py:19-1956 are data tables (vertex positions, normals,
 uvs, face indices).  A drop-in must reproduce them exactly (face order decides
 argmin ties), so this script reads the numbers out of those files and stores
-them as arrays.  It needs the reference checkout; the product only loads the
-resulting .npz.
+them as arrays.  It needs the reference checkout and is run by hand; the
+product only loads the resulting jaxrenderer_amd/shapes/mesh_tables.npz.
 
-    python -m jaxrenderer_amd.shapes.extract_tables /root/reference
+    python tests/golden/extract_tables.py /root/reference
 """
 
 import os
@@ -54,4 +54,5 @@ if __name__ == "__main__":
     tables = extract(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
     for k, v in tables.items():
         print(k, v.shape)
-    np.savez(os.path.join(os.path.dirname(os.path.abspath(__file__)), "mesh_tables.npz"), **tables)
+    root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    np.savez(os.path.join(root, "jaxrenderer_amd", "shapes", "mesh_tables.npz"), **tables)
