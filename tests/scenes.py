@@ -210,6 +210,13 @@ def soup_inputs(shader: type, seed: int, width: int = 48, height: int = 40, n_tr
     if shader is PhongTextureShader:
         return camera, faces, PhongTextureExtraInput(position=pos, normal=normal, uv=uv,
                                                      light=light, texture=tex), None
+    if shader is PhongTextureDarbouxShader:
+        nmap = rng.normal(size=(6, 4, 3)).astype(f32)
+        id_to_face = np.repeat(np.arange(n_tris, dtype=np.int32), 3)
+        slot_faces = np.arange(V, dtype=np.int32).reshape(n_tris, 3)
+        return camera, slot_faces, PhongTextureDarbouxExtraInput(
+            position=pos, normal=normal, uv=uv, light=light, texture=tex, normal_map=nmap,
+            id_to_face=id_to_face, faces_indices=slot_faces), None
     main = calls[-1]
     # the reflection shaders' soup draws its extra arrays after the ones above
     n_obj = 3

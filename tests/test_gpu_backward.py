@@ -95,6 +95,10 @@ def _ref_params(L, extra, shadow_cam):
             P[k] = L[f]
     if hasattr(extra, "uv"):
         P["uv"] = torch.as_tensor(np.asarray(extra.uv), dtype=torch.float64)
+    if hasattr(extra, "normal_map"):
+        P["normal_map"] = torch.as_tensor(np.asarray(extra.normal_map), dtype=torch.float64)
+        P["id_to_face"] = torch.as_tensor(np.asarray(extra.id_to_face), dtype=torch.int64)
+        P["faces_indices"] = torch.as_tensor(np.asarray(extra.faces_indices), dtype=torch.int64)
     if hasattr(extra, "texture_shape"):
         P["tex_shapes"] = torch.as_tensor(np.asarray(extra.texture_shape), dtype=torch.int64)
         P["tex_index"] = torch.as_tensor(np.asarray(extra.texture_index), dtype=torch.int64)

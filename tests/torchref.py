@@ -83,6 +83,40 @@ def forward(shader: int, W: int, H: int, prim_id: np.ndarray, faces: np.ndarray,
         pos = (lc >= 0).all(-1, keepdim=True)
         amb |= near_int(uv).any(-1) | (lc.abs() < 1e-4).any(-1)
         col = torch.where(pos, P["texture"][u, v] * lc, torch.zeros_like(lc))
+    elif shader == _hip.SHADER_PHONG_DARBOUX:
+        # renderer/shaders/phong_darboux.py:116-293: the triangle of the face
+        # of the primitive's first vertex, in NDC, and its uvs; a tangent frame
+        # from inv([p1 - p0; p2 - p0; n]) and the normal map in that frame
+        ne = N(apply(P["eye"], N(N(P["normal"][fc])), 0.0)[..., :3])
+        n = (b[..., None] * ne).sum(1)
+        uv = (b[..., None] * P["uv"][fc]).sum(1)
+        tw, th = P["texture"].shape[:2]
+        u = torch.remainder(torch.floor(uv[:, 0]).long(), tw)
+        v = torch.remainder(torch.floor(uv[:, 1]).long(), th)
+        fs = P["faces_indices"][P["id_to_face"][fc[:, 0]]]  # [K, 3] slots
+        tc = apply(P["w2c"], P["pos"][fs], 1.0)
+        w = tc[..., 3:]
+        tri = torch.where(w == 0, tc[..., :3], tc[..., :3] / torch.where(w == 0, 1.0, w))
+        tuv = P["uv"][fs]
+        nh = N(n)
+        A = torch.stack([tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0], nh], dim=1)
+        AI = torch.linalg.inv(A)
+        zero = torch.zeros_like(uv[:, 0])
+        du = torch.stack([tuv[:, 1, 0] - tuv[:, 0, 0], tuv[:, 2, 0] - tuv[:, 0, 0], zero], -1)
+        dv = torch.stack([tuv[:, 1, 1] - tuv[:, 0, 1], tuv[:, 2, 1] - tuv[:, 0, 1], zero], -1)
+        ti = (AI @ du[..., None])[..., 0]
+        tj = (AI @ dv[..., None])[..., 0]
+        B = torch.stack([N(ti), N(tj), nh], dim=-1)  # columns
+        nm = P["normal_map"]
+        nmv = nm[u.clamp(0, nm.shape[0] - 1), v.clamp(0, nm.shape[1] - 1)]
+        n2 = N((B @ nmv[..., None])[..., 0])
+        d = (n2 * N(P["light_dir"])).sum(-1, keepdim=True)
+        lc = P["light_col"] * d
+        pos = (lc >= 0).all(-1, keepdim=True)
+        # float32 rounding grows with the condition number of the frame matrix
+        cond = torch.linalg.cond(A)
+        amb |= near_int(uv).any(-1) | (lc.abs() < 1e-4).any(-1) | ~(cond < 1e3)
+        col = torch.where(pos, P["texture"][u, v] * lc, torch.zeros_like(lc))
     elif shader in (_hip.SHADER_PHONG_REFLECTION, _hip.SHADER_PHONG_REFLECTION_SHADOW):
         ne = N(apply(P["eye"], N(N(P["normal"][fc])), 0.0)[..., :3])
         n = (b[..., None] * ne).sum(1)
