@@ -402,8 +402,11 @@ typedef struct {
   int32_t slot[3];
 } prim;
 
-int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *scenes,
-                       jr_outputs *out) {
+/* jr_forward_cpu, also counting per pixel the primitives that pass
+ * keep & inside & front-facing (the depth complexity the chooser resolves)
+ * into complexity[scene][width][height] when it is not NULL. */
+int32_t jr_forward_cpu_counted(const jr_frame *fr, const jr_inputs *in, const jr_scene *scenes,
+                               jr_outputs *out, int32_t *complexity) {
   if (!fr || !in || !scenes || !out || !out->zbuffer) return JR_E_INVALID;
   const int W = fr->width, H = fr->height, shader = fr->shader;
   for (int s = 0; s < fr->n_scenes; ++s) {
@@ -438,6 +441,7 @@ int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *
         const float x = ((float)i - vp[3]) / vp[0];
         const float y = ((float)j - vp[7]) / vp[5];
         int32_t idx = 0;
+        int32_t n_cover = 0;
         float best = 0.0f;
         for (int32_t f = 0; f < F; ++f) {
           float e[3];
@@ -453,6 +457,7 @@ int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *
           z = z + e[2] * P[f].z[2];
           z = z * vp[10] + vp[11];
           int front = P[f].det >= 0.0f;
+          n_cover += P[f].keep && inside && front;
           float depth = (P[f].keep && inside && front) ? z : INFINITY;
           if (f == 0) {
             best = depth;
@@ -462,6 +467,7 @@ int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *
             idx = f;
           }
         }
+        if (complexity) complexity[pix] = n_cover;
         int keep = 0;
         float z = 0.0f, b[3] = {0.0f, 0.0f, 0.0f}, col[3] = {0.0f, 0.0f, 0.0f};
         if (F > 0) {
@@ -519,6 +525,11 @@ int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *
     free(P);
   }
   return JR_OK;
+}
+
+int32_t jr_forward_cpu(const jr_frame *fr, const jr_inputs *in, const jr_scene *scenes,
+                       jr_outputs *out) {
+  return jr_forward_cpu_counted(fr, in, scenes, out, NULL);
 }
 
 /* Inverse of one 3x3 matrix (row-major), exported for the oracle's own tests. */

@@ -43,6 +43,9 @@ def lib() -> ctypes.CDLL:
         L.jr_forward_cpu.argtypes = [P(_hip.JrFrame), P(_hip.JrInputs), ctypes.c_void_p,
                                      P(_hip.JrOutputs)]
         L.jr_forward_cpu.restype = ctypes.c_int32
+        L.jr_forward_cpu_counted.argtypes = [P(_hip.JrFrame), P(_hip.JrInputs), ctypes.c_void_p,
+                                             P(_hip.JrOutputs), ctypes.c_void_p]
+        L.jr_forward_cpu_counted.restype = ctypes.c_int32
         L.jr_oracle_inv3.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.jr_oracle_det3.argtypes = [ctypes.c_void_p]
         L.jr_oracle_det3.restype = ctypes.c_float
@@ -65,7 +68,9 @@ def _p(a: Optional[np.ndarray]) -> int:
 
 def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.ndarray] = None,
                prim_id: bool = True, bary: bool = True,
-               display: bool = False) -> Dict[str, np.ndarray]:
+               display: bool = False, complexity: bool = False) -> Dict[str, np.ndarray]:
+    """`complexity`: also return, per pixel, how many primitives pass
+    keep & inside & front-facing (the depth complexity)."""
     L = lib()
     B, W, H = call.n_scenes, call.width, call.height
     if zbuffer is None:
@@ -106,8 +111,9 @@ def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.n
     out.zbuffer, out.canvas, out.prim_id, out.bary = _p(zbuffer), _p(canvas), _p(pid), _p(bar)
     out.display = _p(disp)
     scenes = (_hip.JrScene * B)(*call.scenes)
-    rc = L.jr_forward_cpu(ctypes.byref(fr), ctypes.byref(jin), ctypes.cast(scenes, ctypes.c_void_p),
-                          ctypes.byref(out))
+    cx = np.zeros((B, W, H), dtype=np.int32) if complexity else None
+    rc = L.jr_forward_cpu_counted(ctypes.byref(fr), ctypes.byref(jin),
+                                  ctypes.cast(scenes, ctypes.c_void_p), ctypes.byref(out), _p(cx))
     if rc != 0:
         raise RuntimeError(f"jr_forward_cpu failed: {rc}")
     res = {"zbuffer": zbuffer}
@@ -119,6 +125,8 @@ def run_oracle(call, zbuffer: Optional[np.ndarray] = None, canvas: Optional[np.n
         res["bary"] = bar
     if disp is not None:
         res["display"] = disp
+    if cx is not None:
+        res["complexity"] = cx
     return res
 
 
