@@ -23,6 +23,94 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# fp32 VALU issue rate of the chip without packed or fused operations:
+# 256 CUs x 64 lanes x 2.4 GHz, in Gops/s (the operations of the work model
+# below are unfused fp32 operations, the code being built with contraction off)
+VALU_PEAK_GOPS = 256 * 64 * 2.4
+
+
+def work_model(scenes, width: int, height: int, smspec, sample: dict) -> dict:
+    """Algorithmic work of one frame of a batch, per stage and in total.
+
+    Bytes are the compulsory input and output bytes: 12 B per unique
+    position, 12 B per unique normal, 8 B per unique uv, 36 B per triangle
+    (three int32 index triples), 16 B per texel of every object's diffuse +
+    specular map plus 76 B of per-object pose, 16 B per pixel of colour +
+    depth.  The shadow map is an intermediate and counts nothing.  Ops are
+    unfused fp32 VALU operations: 120 per corner for the main vertex stage,
+    80 per triangle for each pass's setup (M, det, inverse), 28 per unique
+    vertex for the shadow vertex stage, 30 per (pixel, covering primitive)
+    pair and 220 per covered pixel in the main pass, 25 per (texel, covering
+    primitive) pair in the shadow pass.  "Covering" means passing keep,
+    inside and front-facing (the depth complexity); it is counted by the CPU
+    oracle on `sample` (a prefix of the batch) and scaled by the batch's
+    triangle count over the sample's.
+
+    A stage is charged the bytes it must read or write itself, so the two
+    setups each count the positions and faces, which the frame total counts
+    once."""
+    F = Vu = Nu = UVu = maps = 0
+    for s in scenes:
+        m = s.model
+        F += len(m.faces)
+        Vu += len(m.verts)
+        Nu += len(m.norms)
+        UVu += len(m.uvs)
+        ts = m.texture_shape
+        maps += sum(16 * int(w) * int(h) + 76 for w, h in ts)
+    B = len(scenes)
+    scale = F / max(sample["triangles"], 1)
+    P_main = sample["P_main"] * scale
+    C_main = sample["C_main"] * scale
+    P_shadow = sample["P_shadow"] * scale if smspec is not None else 0.0
+    out_b = B * width * height * 16
+    geo_b = 12 * Vu + 36 * F
+    stage_bytes = {"main.setup": geo_b, "main.raster": 12 * Nu + 8 * UVu + maps + out_b}
+    stage_ops = {"main.setup": 120 * 3 * F + 80 * F, "main.raster": 30 * P_main + 220 * C_main}
+    if smspec is not None:
+        stage_bytes.update({"shadow.setup": geo_b, "shadow.raster": 0})
+        stage_ops.update({"shadow.setup": 28 * Vu + 80 * F, "shadow.raster": 25 * P_shadow})
+    total_b = 12 * Vu + 12 * Nu + 8 * UVu + 36 * F + maps + out_b
+    total_ops = sum(stage_ops.values())
+    return {"bytes": stage_bytes, "ops": stage_ops, "bytes_total": total_b,
+            "ops_total": total_ops,
+            "counts": {"scenes": B, "triangles": F, "unique_positions": Vu,
+                       "unique_normals": Nu, "unique_uvs": UVu,
+                       "P_main": round(P_main), "C_main": round(C_main),
+                       "P_shadow": round(P_shadow)}}
+
+
+def fractions(nbytes: float, ops: float, ms: float) -> dict:
+    """Achieved rates of `nbytes` and `ops` done in `ms`, against the HBM and
+    VALU peaks; `bound` names the larger fraction."""
+    t = ms * 1e-3
+    gbs = nbytes / t / 1e9
+    gops = ops / t / 1e9
+    hbm, valu = gbs / HBM_PEAK_GBS, gops / VALU_PEAK_GOPS
+    return {"bound": "valu" if valu >= hbm else "hbm",
+            "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(hbm, 4), "bytes": int(nbytes)},
+            "valu": {"achieved": round(gops, 1), "peak": round(VALU_PEAK_GOPS, 1),
+                     "unit": "Gops/s", "frac": round(valu, 4), "ops": int(ops)}}
+
+
+def complexity_sample(scenes, width: int, height: int, smspec, n: int) -> dict:
+    """Depth complexity of the first n scenes, counted by the CPU oracle."""
+    from jaxrenderer_amd import batch
+    from oracle.oracle import run_oracle_plan
+
+    P_main = C_main = P_shadow = tris = 0
+    for s in scenes[:n]:
+        res = run_oracle_plan(batch.plan_batch([s], width, height, smspec), prim_id=False,
+                              bary=False, complexity=True)
+        cx = res["main"]["complexity"]
+        P_main += int(cx.sum())
+        C_main += int((cx > 0).sum())
+        if "shadow" in res:
+            P_shadow += int(res["shadow"]["complexity"].sum())
+        tris += len(s.model.faces)
+    return {"scenes": min(n, len(scenes)), "triangles": tris, "P_main": P_main,
+            "C_main": C_main, "P_shadow": P_shadow}
 
 
 def log(msg: str) -> None:
@@ -61,6 +149,12 @@ def main() -> None:
                     help="c3 (default): 1024 Brax-like scenes at 84x84; c4: 64-view "
                          "differentiation of a 5000-triangle sphere at 256x256; c5: 512 "
                          "scenes of 1k..20k triangles at 480x270, forward + backward")
+    ap.add_argument("--no-extra-configs", action="store_true",
+                    help="skip the summaries of configs 1, 2, 4 and 5 in the default line")
+    ap.add_argument("--c5-per-scene", type=int, default=64,
+                    help="config 5: scenes timed one by one for the per-scene comparison")
+    ap.add_argument("--complexity-scenes", type=int, default=32,
+                    help="scenes whose depth complexity the oracle counts for the work model")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy, the reference "
                          "kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE")
@@ -76,9 +170,11 @@ def main() -> None:
     from jaxrenderer_amd import _hip, batch, workloads
 
     if args.workload == "c4":
-        return run_views(args)
+        print(json.dumps(run_views(args)), flush=True)
+        return
     if args.workload == "c5":
-        return run_large(args)
+        print(json.dumps(run_large(args)), flush=True)
+        return
 
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -243,8 +339,9 @@ def main() -> None:
         log(f"batched vs per-scene: {bad} of {B} scenes differ")
         del singles
 
-    # CPU oracle on a bounded sample of the same scenes
-    cpu = None
+    # CPU leg: the oracle timed on a bounded sample of the same scenes (the
+    # CPU baseline), and, untimed, the depth complexity the work model needs
+    cpu = sample = None
     if not args.no_cpu_baseline:
         from oracle.oracle import run_oracle_plan
 
@@ -261,6 +358,10 @@ def main() -> None:
                "sample": f"first {n_done} of the {B} scenes (same seed), brute-force C oracle, "
                          f"OpenMP over rows, {t_cpu:.1f}s"}
         log(f"cpu oracle: {n_done} scenes in {t_cpu:.1f}s")
+        t1 = time.perf_counter()
+        sample = complexity_sample(scenes, W, H, smspec, args.complexity_scenes)
+        log(f"depth complexity of {sample['scenes']} scenes counted by the oracle in "
+            f"{time.perf_counter() - t1:.1f}s: {sample}")
 
     # triangles that won at least one pixel of the main pass (their shading
     # inputs are read by the raster stage)
@@ -274,18 +375,18 @@ def main() -> None:
     if not had_ids:
         main_pass.prim_id = None
 
-    # roofline of the dominant kernel: algorithmic bytes of one launch (the
-    # per-unit figures below, also listed in DESIGN.md) over its event-timed
-    # duration on the rendering stream
     n_tri = int(tris.sum())
     px = {"main": B * W * H, "shadow": B * smspec.width * smspec.height}
     tiles = {tag: p.n_tiles for tag, p in br.passes.items()}
 
-    # the shadow pass keeps 32-bit depth keys in its output buffer (a depth
-    # pass with an offset and no id outputs); the main pass 64-bit keys
+    # the implementation's own byte model per stage (what the kernels must
+    # move as built: key buffers, records, lists), beside the algorithmic
+    # bytes.  The shadow pass keeps 32-bit depth keys in its output buffer
+    # (a depth pass with an offset and no id outputs); the main pass 64-bit
+    # keys.
     key_b = {"shadow": 4, "main": 8}
 
-    def alg_bytes(pass_: str, stage: str) -> int:
+    def impl_bytes(pass_: str, stage: str) -> int:
         if stage == "clear":  # key per pixel, tile counters
             return px[pass_] * key_b[pass_] + tiles[pass_] * 8
         if stage == "setup":  # 3 slot ids (+3 position indices in the main
@@ -307,26 +408,50 @@ def main() -> None:
         shade = n_visible * (64 + 12 + 3 * 48) if pass_ == "main" else 0
         return px[pass_] * key_b[pass_] + out + pairs[pass_] * 52 + shade
 
-    dpass, dstage = dom.split(".")
-    dom_bytes = alg_bytes(dpass, dstage)
-    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
-    traffic = valu_busy = None
+    # roofline: the algorithmic work of the work model over event-timed
+    # durations, for the whole frame and for the dominant stage, against
+    # both the HBM and the VALU peak
+    wm = work_model(scenes, W, H, smspec, sample) if sample is not None else None
+    dom_fr = frame_fr = None
+    if wm is not None:
+        dom_fr = fractions(wm["bytes"].get(dom, 0), wm["ops"].get(dom, 0), stages[dom])
+        frame_fr = fractions(wm["bytes_total"], wm["ops_total"], t_batched * 1e3)
+    traffic = valu_busy = prof_tile = None
     tpath = os.path.join(ROOT, "profiles", "config3_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            measured = json.load(f).get("kernels", {})
+            prof = json.load(f)
+        measured = prof.get("kernels", {})
         if dom in measured:
             traffic = measured[dom]["hbm_bytes"]
             valu_busy = measured[dom].get("valu_busy")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": dom, "kernel_ms": round(stages[dom], 4), "alg_bytes": int(dom_bytes),
-                "alg_bytes_by_kernel": {k: alg_bytes(*k.split(".")) for k in stages},
-                "visible_triangles": n_visible,
-                # share of the chip's VALU issue slots the same kernel used
-                # (rocprof counters, profiles/config3_traffic.json): the
-                # per-texel edge arithmetic, not HBM, bounds the raster
-                "valu_busy": valu_busy}
+            prof_tile = measured[dom].get("tile", prof.get("tile"))
+    if dom_fr is not None:
+        best = dom_fr[dom_fr["bound"]]
+        roofline = {"bound": dom_fr["bound"], "achieved": best["achieved"], "peak": best["peak"],
+                    "unit": best["unit"], "frac": best["frac"], "traffic": traffic}
+    else:
+        roofline = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+                    "traffic": traffic}
+    alg_b = wm["bytes"].get(dom, 0) if wm is not None else None
+    roofline.update({
+        "kernel": dom, "kernel_ms": round(stages[dom], 4),
+        "kernel_fractions": dom_fr,
+        "frame_fractions": frame_fr,
+        "work": None if wm is None else {
+            "bytes_by_stage": wm["bytes"], "ops_by_stage": {k: int(v) for k, v in wm["ops"].items()},
+            "bytes_total": int(wm["bytes_total"]), "ops_total": int(wm["ops_total"]),
+            "counts": wm["counts"], "complexity_sample": sample},
+        # HBM bytes the dominant stage's kernel moved per launch, from a
+        # rocprofv3 FETCH_SIZE / WRITE_SIZE run committed under profiles/
+        # (not measured in this run); its launch shape is named beside it
+        "traffic_source": None if traffic is None else
+        f"profiles/config3_traffic.json ({prof_tile or 'launch shape not recorded'})",
+        "traffic_over_alg_bytes": (round(traffic / alg_b, 2) if traffic and alg_b else None),
+        "valu_busy_from_profile": valu_busy,
+        "traffic_model": {k: impl_bytes(*k.split(".")) for k in stages},
+        "visible_triangles": n_visible,
+    })
 
     res = {
         "metric": "frames/s at 84x84 batch 1024 (fwd, fwd+bwd) vs per-scene launch, 1 GPU",
@@ -342,9 +467,10 @@ def main() -> None:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "config 3: 1024 Brax-like scenes (ground plane + 4..30 boxes/capsules/"
-                        "spheres, 650..10k triangles), 84x84, phong_reflection_shadow, "
-                        "256x256 shadow map, batched forward in one HIP-graph replay",
+            "workload": f"config 3: {B} Brax-like scenes (ground plane + 4..30 boxes/"
+                        f"capsules/spheres, {int(tris.min())}..{int(tris.max())} triangles), "
+                        "84x84, phong_reflection_shadow, 256x256 shadow map, batched forward "
+                        "in one HIP-graph replay",
             "scenes": B,
             "triangles": n_tri,
             "width": W,
@@ -368,7 +494,16 @@ def main() -> None:
         "binned_pairs": pairs,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "other_configs": None,
     }
+    if not args.no_extra_configs:
+        import gc
+
+        del br, out, main_pass, pid
+        gc.collect()
+        torch.cuda.empty_cache()
+        sub = argparse.Namespace(**{**vars(args), "c5_per_scene": min(args.c5_per_scene, 16)})
+        res["other_configs"] = other_configs(sub, cpu_sample=not args.no_cpu_baseline)
     print(json.dumps(res), flush=True)
 
 
@@ -385,7 +520,71 @@ def _timed(fn, steps: int, warmup: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
-def run_views(args) -> None:
+def single_scene(args, scene, width: int, height: int, cpu_sample: bool) -> dict:
+    """One scene's plain forward (phong_reflection, no shadow) replayed from a
+    HIP graph, and the CPU oracle on the same frame."""
+    from jaxrenderer_amd import batch
+
+    calls = batch.plan_batch([scene], width, height)
+    br = batch.BatchRenderer(calls, tile=(8, 8), persistent=False)
+    if args.tile == "auto":
+        br.autotune(use_cache=not args.no_autotune_cache)
+    br.calibrate()
+    br.capture()
+    t = _timed(br.graph.replay, args.steps, args.warmup)
+    d = {"triangles": int(len(scene.model.faces)), "width": width, "height": height,
+         "fwd_ms": round(t * 1e3, 4), "frames_per_s": round(1.0 / t, 1),
+         "mpix_per_s": round(width * height / t / 1e6, 2),
+         "tile": list(br.passes["main"].tile)}
+    if cpu_sample:
+        from oracle.oracle import run_oracle_plan
+
+        t1 = time.perf_counter()
+        run_oracle_plan(calls, prim_id=False, bary=False)
+        tc = time.perf_counter() - t1
+        d["cpu_oracle_frames_per_s"] = round(1.0 / tc, 3)
+    return d
+
+
+def other_configs(args, cpu_sample: bool) -> dict:
+    """Short summaries of the other workloads, measured in the same run:
+    config 1 (the cube, 320x240) and config 2 (the 9k-triangle sphere,
+    960x540) single-scene forwards; config 4 (64 views, forward and forward +
+    backward per shader); config 5 (512 scenes at 480x270, forward, forward +
+    backward, peak HBM, per-scene forward + backward on a prefix)."""
+    import gc
+
+    import torch
+
+    from jaxrenderer_amd import workloads
+
+    out = {}
+    t0 = time.time()
+    out["config1"] = single_scene(args, workloads.config1(), 320, 240, cpu_sample)
+    out["config2"] = single_scene(args, workloads.config2(seed=args.seed), 960, 540, cpu_sample)
+    log(f"configs 1 and 2: {out['config1']['fwd_ms']:.4f} / {out['config2']['fwd_ms']:.4f} ms "
+        f"per frame ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    c4 = run_views(args)
+    out["config4"] = {k: {kk: v[kk] for kk in ("fwd_ms", "fwd_bwd_ms", "fwd_bwd_over_fwd",
+                                                "fwd_views_per_s", "fwd_bwd_views_per_s",
+                                                "grads_finite")}
+                      for k, v in c4["shaders"].items()}
+    log(f"config 4 summary in {time.time() - t0:.1f}s")
+    gc.collect()
+    torch.cuda.empty_cache()
+    t0 = time.time()
+    c5 = run_large(args)
+    out["config5"] = {k: c5[k] for k in ("value", "unit", "ms_per_step", "fwd_fps", "fwd_ms",
+                                         "fwd_bwd_over_fwd", "peak_hbm_bytes",
+                                         "per_scene_fwd_bwd_fps", "batched_over_per_scene",
+                                         "grads_finite")}
+    out["config5"]["triangles"] = c5["config"]["triangles"]
+    log(f"config 5 summary in {time.time() - t0:.1f}s")
+    return out
+
+
+def run_views(args) -> dict:
     """64 ring views of a deformed 5000-triangle sphere at 256x256 (512x512
     texture), gouraud_texture and phong: batched forward, and forward + L2
     loss gradient + backward to vertices, normals, texture and the per-view
@@ -434,7 +633,7 @@ def run_views(args) -> None:
         log(f"{name}: forward {t_f * 1e3:.3f} ms, forward+backward {t_fb * 1e3:.3f} ms "
             f"for {V} views")
     first = lines["GouraudTextureShader"]
-    print(json.dumps({
+    return ({
         "metric": "views/s, 64 views 256x256 forward+backward (differentiation workload), 1 GPU",
         "value": first["fwd_bwd_views_per_s"], "unit": "views/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": first["fwd_bwd_ms"],
@@ -444,10 +643,10 @@ def run_views(args) -> None:
                                "256x256, 512x512x3 texture, L2 loss against a perturbed "
                                "target (vertex sigma 0.02, texture sigma 0.1)",
                    "views": V, "width": S, "height": S, "triangles": int(len(faces))},
-        "shaders": lines}), flush=True)
+        "shaders": lines})
 
 
-def run_large(args) -> None:
+def run_large(args) -> dict:
     """512 scenes of 1k..20k triangles at 480x270, phong_reflection_shadow with
     512x512 shadow maps: batched forward, batched forward + backward (only the
     primitive ids are kept; fragments are recomputed), peak HBM, and the plain
@@ -465,6 +664,7 @@ def run_large(args) -> None:
         f"{time.time() - t0:.1f}s")
     calls = batch.plan_batch(scenes, W, H, smspec)
     torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
     br = batch.BatchRenderer(calls, persistent=False)
     if args.tile == "auto":
         for tag, c in br.autotune(use_cache=not args.no_autotune_cache).items():
@@ -478,13 +678,15 @@ def run_large(args) -> None:
     br.enable_backward()
     br.capture_train(target)
     t_fb = _timed(br.train_graph.replay, args.steps, args.warmup)
-    peak = torch.cuda.max_memory_allocated()
+    # device memory this workload added at its peak (what was allocated
+    # before it, e.g. by an earlier leg of the same run, is not counted)
+    peak = torch.cuda.max_memory_allocated() - base
     finite = bool(all(torch.isfinite(g).all() for g in br.grads.values()))
     log(f"forward {t_f * 1e3:.2f} ms, forward+backward {t_fb * 1e3:.2f} ms, peak HBM "
         f"{peak / 2**30:.2f} GiB")
     per_scene = None
     if not args.no_per_scene:
-        n_single = min(B, 64)
+        n_single = min(B, args.c5_per_scene)
         singles = []
         for i in range(n_single):
             r = batch.BatchRenderer(batch.plan_batch([scenes[i]], W, H, smspec),
@@ -501,7 +703,7 @@ def run_large(args) -> None:
         t_s = _timed(step, 1, 1)
         per_scene = n_single / t_s
         log(f"per-scene forward+backward: {per_scene:.1f} frames/s over {n_single} scenes")
-    print(json.dumps({
+    return ({
         "metric": "frames/s at 480x270 batch 512 (fwd+bwd) vs per-scene launch, 1 GPU",
         "value": round(B / t_fb, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t_fb * 1e3, 4), "higher_is_better": True,
@@ -514,7 +716,7 @@ def run_large(args) -> None:
         "fwd_bwd_over_fwd": round(t_f / t_fb, 3), "peak_hbm_bytes": int(peak),
         "per_scene_fwd_bwd_fps": None if per_scene is None else round(per_scene, 1),
         "batched_over_per_scene": None if per_scene is None else round(B / t_fb / per_scene, 2),
-        "grads_finite": finite, "binned_pairs": pairs}), flush=True)
+        "grads_finite": finite, "binned_pairs": pairs})
 
 
 if __name__ == "__main__":

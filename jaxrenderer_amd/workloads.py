@@ -139,6 +139,20 @@ def config3(n_scenes: int = 1024, seed: int = 0, width: int = 84, height: int = 
     return scenes, ShadowMapSpec(width=shadow_size, height=shadow_size)
 
 
+def config1(width: int = 320, height: int = 240) -> BatchScene:
+    """The cube of create_cube(half_extents=1, texture_scaling=1) with a pure
+    blue 2x2 diffuse map and specular 2.0, seen from (2, 4, 1) looking at the
+    origin (up +z, vfov 60, square pixels), default light, phong_reflection."""
+    hfov = float(np.degrees(2 * math.atan(math.tan(math.radians(30)) * width / height)))
+    cp = CameraParameters(viewWidth=width, viewHeight=height, position=(2.0, 4.0, 1.0),
+                          target=(0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0), vfov=60.0, hfov=hfov)
+    diffuse = np.zeros((2, 2, 3), f32)
+    diffuse[..., 2] = 1.0
+    model = create_cube(np.ones(3, f32), np.ones(2, f32), diffuse, np.full((2, 2), 2.0, f32))
+    return BatchScene(model=merge_objects([ModelObject(model=model)]),
+                      camera=Renderer.create_camera_from_parameters(cp), light=LightParameters())
+
+
 def config2(seed: int = 0, width: int = 960, height: int = 540) -> BatchScene:
     rng = _rng(seed)
     n_lat, n_lon = 48, 96
@@ -240,5 +254,5 @@ def triangle_counts(scenes: Sequence[BatchScene]) -> np.ndarray:
     return np.array([len(s.model.faces) for s in scenes])
 
 
-__all__ = ["config2", "config3", "config4", "config5", "brax_like_objects",
+__all__ = ["config1", "config2", "config3", "config4", "config5", "brax_like_objects",
            "dense_brax_like_objects", "uv_sphere", "NOTEBOOK_LIGHT"]
