@@ -224,6 +224,23 @@ int32_t jr_forward(const jr_frame* frame, const jr_inputs* inputs,
                    const jr_scene* scenes, jr_outputs* outputs, void* workspace,
                    int64_t workspace_bytes, jr_stream_t stream);
 
+/* jr_forward without a workspace argument and without any allocation: no
+ * counterpart in the reference (JAX allocates its own temporaries).  It is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * workspace is the library's own per-GPU scratch array of JR_SCRATCH_BYTES,
+ * resident from the library's load to the end of the process; a frame whose
+ * jr_workspace_size exceeds it is rejected with JR_E_WORKSPACE before any
+ * launch.  Whole passes only (frame->part must be JR_PART_ALL); stage timing
+ * is never recorded.  Every call on a GPU shares that GPU's scratch: calls
+ * that overlap in time on one GPU (two streams, two threads) race and must be
+ * serialised by the caller.  Each call clears what it reads before writing
+ * it, so no state carries over from the previous call. */
+#define JR_SCRATCH_BYTES (64LL << 20)
+int32_t jr_forward_noalloc(const jr_frame* frame, const jr_inputs* inputs,
+                           const jr_scene* scenes, jr_outputs* outputs,
+                           jr_stream_t stream);
+
 /* Number of triangle-tile pairs the last jr_forward on `workspace` binned
  * (device value, read asynchronously into *pairs on `stream`; the caller
  * synchronises).  Used to size bin_capacity. */

@@ -161,6 +161,9 @@ class JrPoseArgs(ctypes.Structure):
     ]
 
 
+# bytes of jr_forward_noalloc's per-GPU scratch (include/jrhip.h)
+SCRATCH_BYTES = 64 << 20
+
 # largest normal count of one object jr_pose accepts (include/jrhip.h)
 POSE_MAX_NORMALS = 43690
 
@@ -195,6 +198,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.jr_workspace_size.restype = c_i64
     lib.jr_forward.argtypes = [P(JrFrame), P(JrInputs), c_vp, P(JrOutputs), c_vp, c_i64, c_vp]
     lib.jr_forward.restype = c_i32
+    lib.jr_forward_noalloc.argtypes = [P(JrFrame), P(JrInputs), c_vp, P(JrOutputs), c_vp]
+    lib.jr_forward_noalloc.restype = c_i32
     lib.jr_binned_pairs.argtypes = [c_vp, P(JrFrame), c_i32, P(c_i64), c_vp]
     lib.jr_binned_pairs.restype = c_i32
     lib.jr_stage_timing.argtypes = [c_i32]

@@ -104,9 +104,10 @@ def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         canvas: Optional[torch.Tensor] = None, prim_id: bool = False, bary: bool = False,
         display: bool = False, tile=_raster.DEFAULT_TILE, persistent: bool = False,
         tiles_per_group: int = 1, pixels_per_lane: int = 1,
-        capacity: Optional[int] = None) -> Dict[str, torch.Tensor]:
+        capacity: Optional[int] = None, noalloc: bool = False) -> Dict[str, torch.Tensor]:
     """Executes `call` on the GPU.  zbuffer/canvas given here are updated in
-    place (merge semantics when call.init_buffers is False)."""
+    place (merge semantics when call.init_buffers is False).  `noalloc`: run
+    through jr_forward_noalloc (the library's scratch as workspace)."""
     dev = _raster.require_device()
     B, W, H = call.n_scenes, call.width, call.height
     if zbuffer is None:
@@ -140,6 +141,7 @@ def run(call: RasterCall, *, zbuffer: Optional[torch.Tensor] = None,
         colour_default=call.colour_default, apply_offset=call.apply_offset,
         depth_offset=call.depth_offset, tile=tile, persistent=persistent,
         tiles_per_group=tiles_per_group, pixels_per_lane=pixels_per_lane, capacity=capacity,
+        noalloc=noalloc,
     )
     return out
 

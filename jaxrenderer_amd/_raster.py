@@ -221,11 +221,14 @@ def forward(
     stream: Optional[torch.cuda.Stream] = None,
     workspace: Optional[torch.Tensor] = None,
     part: int = _hip.PART_ALL,
+    noalloc: bool = False,
 ) -> None:
     """One jr_forward call on `stream` (torch's current stream by default).
     `part`: every stage, or only the binning stages (PART_BIN) or only the
     raster (PART_RASTER, on the workspace of a PART_BIN call of the same
-    frame)."""
+    frame).  `noalloc`: call jr_forward_noalloc instead, whose workspace is
+    the library's own per-GPU scratch (no allocation at all; frames up to
+    SCRATCH_BYTES of workspace, whole passes only)."""
     L = _hip.lib()
     dev = zbuffer.device
     if dev.type != "cuda":
@@ -256,6 +259,12 @@ def forward(
         raise ValueError("scenes must be a uint8 device tensor of jr_scene structs")
     if scenes.numel() != n_scenes * ctypes.sizeof(_hip.JrScene):
         raise ValueError("scenes tensor size does not match n_scenes")
+    if noalloc:
+        rc = L.jr_forward_noalloc(ctypes.byref(fr), ctypes.byref(jin),
+                                  ctypes.c_void_p(scenes.data_ptr()), ctypes.byref(jout),
+                                  ctypes.c_void_p(stream.cuda_stream))
+        _hip.check(rc, "jr_forward_noalloc")
+        return
     need = L.jr_workspace_size(ctypes.byref(fr), int(n_tris))
     if need < 0:
         _hip.check(-1, "jr_workspace_size")

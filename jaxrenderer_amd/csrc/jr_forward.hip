@@ -1461,19 +1461,17 @@ int64_t jr_workspace_size(const jr_frame* frame, int32_t n_tris) {
   return ws_layout(frame, n_tris).total;
 }
 
-int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
-                   jr_outputs* out, void* workspace, int64_t workspace_bytes,
-                   jr_stream_t stream_) {
-  int32_t rc = check_frame(fr);
-  if (rc) return rc;
-  rc = check_inputs(fr, in, out);
-  if (rc) return rc;
-  if (!scenes) return fail(JR_E_INVALID, "scenes is NULL");
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream_);
-  const WsLayout L = ws_layout(fr, in->n_tris);
-  if (!workspace || workspace_bytes < L.total)
-    return fail(JR_E_WORKSPACE, "workspace smaller than jr_workspace_size()");
-  char* ws = static_cast<char*>(workspace);
+}  // extern "C"
+
+namespace jr {
+
+// The body of jr_forward and jr_forward_noalloc: validated arguments, a
+// workspace of at least ws_layout(fr, n_tris).total bytes.  `timed`: record
+// the stage events when jr_stage_timing is on (creating events allocates, so
+// the no-allocation entry never times).
+int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                     jr_outputs* out, char* ws, const WsLayout& L, hipStream_t st,
+                     bool timed) {
   Frame f;
   f.W = fr->width;
   f.H = fr->height;
@@ -1513,7 +1511,7 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   uint32_t* wbits = reinterpret_cast<uint32_t*>(ws + L.wbits);
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
-  StageTimer timer(st, fr->part == JR_PART_ALL);
+  StageTimer timer(st, timed && fr->part == JR_PART_ALL);
   const int n_tris = in->n_tris;
   if (fr->part != JR_PART_RASTER) {
   // counts, cursor and misc are contiguous: clear them, and the key buffer,
@@ -1596,6 +1594,76 @@ int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scen
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(JR_E_HIP, std::string("launch failed: ") + hipGetErrorString(err));
   return JR_OK;
+}
+
+// Scratch of jr_forward_noalloc: one array per GPU, resident from the
+// library's load until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it names a stand-in, not device memory): the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current, and is kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[JR_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int32_t scratch_of_current_device(char** ptr) {
+  int dev = -1;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return fail(JR_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(err));
+  if (dev < 0 || dev >= kMaxDevices) return fail(JR_E_INVALID, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p)
+      return fail(JR_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                (err != hipSuccess ? hipGetErrorString(err) : "NULL address"));
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = static_cast<char*>(g_scratch_ptr[dev]);
+  return JR_OK;
+}
+
+}  // namespace jr
+
+extern "C" {
+
+int32_t jr_forward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                   jr_outputs* out, void* workspace, int64_t workspace_bytes,
+                   jr_stream_t stream_) {
+  int32_t rc = check_frame(fr);
+  if (rc) return rc;
+  rc = check_inputs(fr, in, out);
+  if (rc) return rc;
+  if (!scenes) return fail(JR_E_INVALID, "scenes is NULL");
+  const WsLayout L = ws_layout(fr, in->n_tris);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(JR_E_WORKSPACE, "workspace smaller than jr_workspace_size()");
+  return forward_body(fr, in, scenes, out, static_cast<char*>(workspace), L,
+                      reinterpret_cast<hipStream_t>(stream_), true);
+}
+
+int32_t jr_forward_noalloc(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                           jr_outputs* out, jr_stream_t stream_) {
+  int32_t rc = check_frame(fr);
+  if (rc) return rc;
+  rc = check_inputs(fr, in, out);
+  if (rc) return rc;
+  if (!scenes) return fail(JR_E_INVALID, "scenes is NULL");
+  if (fr->part != JR_PART_ALL)
+    return fail(JR_E_INVALID, "jr_forward_noalloc runs whole passes only (part = JR_PART_ALL)");
+  const WsLayout L = ws_layout(fr, in->n_tris);
+  if (L.total > (int64_t)JR_SCRATCH_BYTES)
+    return fail(JR_E_WORKSPACE,
+                "frame needs " + std::to_string(L.total) + " bytes of workspace; "
+                "jr_forward_noalloc's scratch holds JR_SCRATCH_BYTES = " +
+                std::to_string((long long)JR_SCRATCH_BYTES) + " (use jr_forward)");
+  char* ws = nullptr;
+  rc = scratch_of_current_device(&ws);
+  if (rc) return rc;
+  return forward_body(fr, in, scenes, out, ws, L, reinterpret_cast<hipStream_t>(stream_),
+                      false);
 }
 
 int32_t jr_stage_timing(int32_t enable) {
