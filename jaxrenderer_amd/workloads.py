@@ -203,15 +203,21 @@ def dense_brax_like_objects(rng, target_tris: int) -> List[ModelObject]:
 
 
 def config5(n_scenes: int = 512, seed: int = 0, width: int = 480, height: int = 270,
-            shadow_size: int = 512, min_tris: int = 1000, max_tris: int = 20000
+            shadow_size: int = 512, min_tris: int = 1000, max_tris: int = 20000,
+            targets: Optional[Sequence[int]] = None
             ) -> Tuple[List[BatchScene], ShadowMapSpec]:
     """Batched forward + backward workload: Brax-like scenes of 1k..20k
     triangles (target drawn log-uniformly per scene) at 480x270 with
-    phong_reflection_shadow and a 512x512 shadow map."""
+    phong_reflection_shadow and a 512x512 shadow map.  `targets`: the
+    triangle targets of the scenes instead (one scene each)."""
     rng = _rng(seed)
     scenes = []
+    if targets is not None:
+        n_scenes = len(targets)
     for i in range(n_scenes):
         target = int(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris))))
+        if targets is not None:
+            target = int(targets[i])
         model = merge_objects(dense_brax_like_objects(rng, target))
         scenes.append(BatchScene(model=model, camera=ring_camera(i, n_scenes, width, height),
                                  light=NOTEBOOK_LIGHT))

@@ -43,8 +43,9 @@ def forward(shader: int, W: int, H: int, prim_id: np.ndarray, faces: np.ndarray,
     t = pid[ii, jj]
     fc = torch.as_tensor(faces, dtype=torch.int64)[t]  # [K, 3] slots
     vp = P["vp"]
-    x = (ii.double() - vp[0, 3]) / vp[0, 0]
-    y = (jj.double() - vp[1, 3]) / vp[1, 1]
+    dt = P["pos"].dtype  # float64 for reference gradients; float32 to measure rounding
+    x = (ii.to(dt) - vp[0, 3]) / vp[0, 0]
+    y = (jj.to(dt) - vp[1, 3]) / vp[1, 1]
     clip = apply(P["w2c"], P["pos"][fc], 1.0)  # [K, 3, 4]
     M = torch.stack([clip[..., 0], clip[..., 1], clip[..., 3]], dim=-1)
     Mi = torch.linalg.inv(M)
@@ -54,7 +55,7 @@ def forward(shader: int, W: int, H: int, prim_id: np.ndarray, faces: np.ndarray,
     z = (e * clip[..., 2]).sum(-1) * vp[2, 2] + vp[2, 3]
     amb = torch.zeros_like(x, dtype=torch.bool)
     K = len(t)
-    col = torch.zeros((K, 3), dtype=torch.float64)
+    col = torch.zeros((K, 3), dtype=dt)
     if shader in (_hip.SHADER_GOURAUD, _hip.SHADER_GOURAUD_TEXTURE):
         L = N(P["light_dir"])
         inten = (N(P["normal"][fc]) * L).sum(-1, keepdim=True)  # [K, 3, 1]
@@ -122,10 +123,11 @@ def forward(shader: int, W: int, H: int, prim_id: np.ndarray, faces: np.ndarray,
         n = (b[..., None] * ne).sum(1)
         uv = (b[..., None] * P["uv"][fc]).sum(1)
         ti = P["tex_index"][fc[:, 0]]
-        shp = P["tex_shapes"][ti.clamp(0, len(P["tex_shapes"]) - 1)].double()
+        shp = P["tex_shapes"][ti.clamp(0, len(P["tex_shapes"]) - 1)]
         fr = uv - torch.where(uv >= 0, torch.floor(uv), torch.ceil(uv))
         fr = torch.where(fr < 0, fr + 1, fr)
-        r0 = fr[:, 0] * shp[:, 0] + (ti * int(P["tex_offset"])).double()
+        shp = shp.to(dt)
+        r0 = fr[:, 0] * shp[:, 0] + (ti * int(P["tex_offset"])).to(dt)
         r1 = fr[:, 1] * shp[:, 1]
         amb |= near_int(r0) | near_int(r1)
         tw, th = P["texture"].shape[:2]
@@ -166,8 +168,8 @@ def forward(shader: int, W: int, H: int, prim_id: np.ndarray, faces: np.ndarray,
             shade = torch.where(lit[:, None], torch.ones_like(P["strength"]).expand(K, 3),
                                 (1.0 - P["strength"]).expand(K, 3))
             col = P["ambient"] * tex + shade * k * tex * P["light_col"]
-    canvas = torch.zeros((W, H, 3), dtype=torch.float64)
-    zbuf = torch.zeros((W, H), dtype=torch.float64)
+    canvas = torch.zeros((W, H, 3), dtype=dt)
+    zbuf = torch.zeros((W, H), dtype=dt)
     canvas = canvas.index_put((ii, jj), col)
     zbuf = zbuf.index_put((ii, jj), z)
     ambiguous = torch.zeros((W, H), dtype=torch.bool)
