@@ -145,6 +145,9 @@ def main() -> None:
     ap.add_argument("--no-per-scene", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
     ap.add_argument("--no-pose", action="store_true", help="skip timing the pose update")
+    ap.add_argument("--full-shadow-map", action="store_true",
+                    help="rasterise every shadow-map texel (default: the main pass computes "
+                         "the texels it reads from the shadow pass's binning)")
     ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
                     help="c3 (default): 1024 Brax-like scenes at 84x84; c4: 64-view "
                          "differentiation of a 5000-triangle sphere at 256x256; c5: 512 "
@@ -202,7 +205,8 @@ def main() -> None:
     auto = args.tile == "auto"
     tile = (8, 8) if auto else wh(args.tile)
     shadow_tile = wh(args.shadow_tile) if args.shadow_tile else tile
-    br = batch.BatchRenderer(calls, tile=tile, shadow_tile=shadow_tile, persistent=True)
+    br = batch.BatchRenderer(calls, tile=tile, shadow_tile=shadow_tile, persistent=True,
+                             shadow_on_demand="auto" if not args.full_shadow_map else False)
     tuned = None
     if auto:
         t0 = time.time()
@@ -216,7 +220,8 @@ def main() -> None:
                     for t in sorted(c["timings"], key=lambda t: t["ms"])))
         log(f"autotune took {time.time() - t0:.1f}s")
     pairs = br.calibrate()
-    log(f"binned triangle-tile pairs: {pairs}")
+    br_on_demand = br.shadow_on_demand
+    log(f"binned triangle-tile pairs: {pairs}; shadow texels on demand: {br_on_demand}")
     br.capture()
     for _ in range(args.warmup):
         br.replay()
@@ -481,6 +486,8 @@ def main() -> None:
                            "pixels_per_lane": p.pixels_per_lane}
                      for tag, p in br.passes.items()},
             "autotuned": auto,
+            "shadow_texels": "on demand (main raster computes the texels it reads from the "
+                             "shadow binning)" if br_on_demand else "full map",
         },
         "mpix_per_s": round(fps * W * H / 1e6, 2),
         "per_scene_fps": None if per_scene_fps is None else round(per_scene_fps, 1),
@@ -578,7 +585,7 @@ def other_configs(args, cpu_sample: bool) -> dict:
     out["config5"] = {k: c5[k] for k in ("value", "unit", "ms_per_step", "fwd_fps", "fwd_ms",
                                          "fwd_bwd_over_fwd", "peak_hbm_bytes",
                                          "per_scene_fwd_bwd_fps", "batched_over_per_scene",
-                                         "grads_finite")}
+                                         "grads_finite", "shadow_texels_on_demand")}
     out["config5"]["triangles"] = c5["config"]["triangles"]
     log(f"config 5 summary in {time.time() - t0:.1f}s")
     return out
@@ -665,7 +672,8 @@ def run_large(args) -> dict:
     calls = batch.plan_batch(scenes, W, H, smspec)
     torch.cuda.reset_peak_memory_stats()
     base = torch.cuda.memory_allocated()
-    br = batch.BatchRenderer(calls, persistent=False)
+    br = batch.BatchRenderer(calls, persistent=False,
+                             shadow_on_demand="auto" if not args.full_shadow_map else False)
     if args.tile == "auto":
         for tag, c in br.autotune(use_cache=not args.no_autotune_cache).items():
             log(f"autotune {tag}: tile {c['tile']} form {c['form']} x{c['ppl']}")
@@ -716,7 +724,8 @@ def run_large(args) -> dict:
         "fwd_bwd_over_fwd": round(t_f / t_fb, 3), "peak_hbm_bytes": int(peak),
         "per_scene_fwd_bwd_fps": None if per_scene is None else round(per_scene, 1),
         "batched_over_per_scene": None if per_scene is None else round(B / t_fb / per_scene, 2),
-        "grads_finite": finite, "binned_pairs": pairs})
+        "grads_finite": finite, "binned_pairs": pairs,
+        "shadow_texels_on_demand": br.shadow_on_demand})
 
 
 if __name__ == "__main__":

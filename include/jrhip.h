@@ -120,6 +120,10 @@ typedef struct jr_inputs {
   int32_t n_tris;
   int32_t shadow_w;
   int32_t shadow_h;
+  /* optional HOST pointer (phong_reflection_shadow only): look the shadow
+   * depths up from the shadow pass's binning instead of reading shadow_maps
+   * (see jr_shadow_source below); NULL: read shadow_maps */
+  const struct jr_shadow_source* shadow_source;
 } jr_inputs;
 
 /* Output buffers, [n_scenes][width][height](,3).  Pixels whose final `keep`
@@ -171,6 +175,32 @@ enum {
   JR_PART_RASTER = 2
 };
 
+/* Shadow depths on demand.  The shadow-map pass of renderer/shadow.py:40-125
+ * fills every texel of the map, but Shadow.get (renderer/shadow.py:127-153)
+ * reads only the texels that shaded pixels project to.  With
+ * jr_inputs.shadow_source set, a phong_reflection_shadow pass computes the
+ * depth of each texel it reads from the shadow pass's binning -- its
+ * small-triangle keys and tile lists, with the same per-texel arithmetic,
+ * chooser, fallback and offset as that pass's raster stage -- so the shadow
+ * pass runs only its binning stages (jr_frame.part = JR_PART_BIN).  Each depth
+ * found is also written to texels[scene][x][y], so the map is right at every
+ * texel a pixel read: jr_backward, given shadow_maps = texels, reads the same
+ * values.  The shadow pass must be a depth-only pass whose small-triangle keys
+ * stay in its zbuffer: JR_SHADER_DEPTH, init_buffers = 1, apply_offset = 1, a
+ * depth_offset other than -0, no prim_id / bary outputs; and its JR_PART_BIN
+ * call must have run on `workspace` and `keys` (stream ordered before this
+ * call).  All pointers but `frame` are device pointers. */
+typedef struct jr_shadow_source {
+  const jr_frame* frame;    /* the shadow pass's frame (host) */
+  const jr_scene* scenes;   /* its n_scenes scene records */
+  const void* workspace;    /* its workspace */
+  int64_t workspace_bytes;
+  const float* keys;        /* its zbuffer after JR_PART_BIN: per-texel 32-bit keys */
+  float* texels;            /* [n_scenes][frame->width][frame->height], written */
+  int32_t n_tris;           /* its triangle count */
+  int32_t pad;
+} jr_shadow_source;
+
 /* Gradient buffers for jr_backward (all optional, accumulated with +=). */
 typedef struct jr_grads {
   float* d_positions;   /* [*,3] scatter-added through pos_index */
@@ -206,7 +236,7 @@ const char* jr_last_error(void);
 
 /* sizeof of the ABI structs, for binding-side layout checks:
  * 0 jr_scene, 1 jr_inputs, 2 jr_outputs, 3 jr_frame, 4 jr_grads,
- * 5 jr_pose_args. */
+ * 5 jr_pose_args, 6 jr_shadow_source. */
 int64_t jr_sizeof(int32_t which);
 
 /* Bytes of workspace jr_forward needs for this frame (bin_capacity included).

@@ -97,6 +97,7 @@ class JrInputs(ctypes.Structure):
         ("n_tris", c_i32),
         ("shadow_w", c_i32),
         ("shadow_h", c_i32),
+        ("shadow_source", c_vp),  # host pointer to a JrShadowSource, or NULL
     ]
 
 
@@ -128,6 +129,21 @@ class JrFrame(ctypes.Structure):
         ("tiles_per_group", c_i32),
         ("pixels_per_lane", c_i32),
         ("part", c_i32),
+    ]
+
+
+class JrShadowSource(ctypes.Structure):
+    """jr_shadow_source: a shadow pass's binning, read by a
+    phong_reflection_shadow pass that computes the shadow depths it needs."""
+    _fields_ = [
+        ("frame", c_vp),  # host pointer to the shadow pass's JrFrame
+        ("scenes", c_vp),
+        ("workspace", c_vp),
+        ("workspace_bytes", c_i64),
+        ("keys", c_vp),
+        ("texels", c_vp),
+        ("n_tris", c_i32),
+        ("pad", c_i32),
     ]
 
 
@@ -181,7 +197,7 @@ DSCENE = {
     "shadow_strength": 66,
 }
 
-STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads, JrPoseArgs)
+STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads, JrPoseArgs, JrShadowSource)
 
 
 class HipError(RuntimeError):

@@ -222,13 +222,16 @@ def forward(
     workspace: Optional[torch.Tensor] = None,
     part: int = _hip.PART_ALL,
     noalloc: bool = False,
+    shadow_source: Optional["ShadowSource"] = None,
 ) -> None:
     """One jr_forward call on `stream` (torch's current stream by default).
     `part`: every stage, or only the binning stages (PART_BIN) or only the
     raster (PART_RASTER, on the workspace of a PART_BIN call of the same
     frame).  `noalloc`: call jr_forward_noalloc instead, whose workspace is
     the library's own per-GPU scratch (no allocation at all; frames up to
-    SCRATCH_BYTES of workspace, whole passes only)."""
+    SCRATCH_BYTES of workspace, whole passes only).  `shadow_source`
+    (phong_reflection_shadow): compute the shadow depths from that shadow
+    pass's binning instead of reading inputs["shadow_maps"]."""
     L = _hip.lib()
     dev = zbuffer.device
     if dev.type != "cuda":
@@ -247,6 +250,10 @@ def forward(
         setattr(jin, name, _ptr(inputs.get(name), name, dt, dev))
     jin.n_slots, jin.n_tris = int(n_slots), int(n_tris)
     jin.shadow_w, jin.shadow_h = int(shadow_shape[0]), int(shadow_shape[1])
+    src = None
+    if shadow_source is not None:
+        src, _keep = shadow_source.struct(dev)
+        jin.shadow_source = ctypes.addressof(src)
     jout = _hip.JrOutputs()
     jout.zbuffer = _ptr(zbuffer, "zbuffer", torch.float32, dev)
     jout.canvas = _ptr(canvas, "canvas", torch.float32, dev)
@@ -275,6 +282,34 @@ def forward(
                       ctypes.byref(jout), ctypes.c_void_p(ws.data_ptr()), ctypes.c_int64(ws.numel()),
                       ctypes.c_void_p(stream.cuda_stream))
     _hip.check(rc, "jr_forward")
+
+
+@dataclass
+class ShadowSource:
+    """A shadow pass (depth shader, offset, no id outputs) whose binning stages
+    ran on `workspace` with `keys` as its zbuffer, for a phong_reflection_shadow
+    pass that computes the texel depths it reads (jr_shadow_source).  `texels`
+    [n_scenes, width, height] receives every depth so computed."""
+    frame: _hip.JrFrame
+    scenes: torch.Tensor
+    workspace: torch.Tensor
+    keys: torch.Tensor
+    texels: torch.Tensor
+    n_tris: int
+
+    def struct(self, dev: torch.device):
+        s = _hip.JrShadowSource()
+        s.frame = ctypes.addressof(self.frame)
+        s.scenes = _ptr(self.scenes, "shadow scenes", torch.uint8, dev)
+        s.workspace = _ptr(self.workspace, "shadow workspace", torch.uint8, dev)
+        s.workspace_bytes = int(self.workspace.numel())
+        s.keys = _ptr(self.keys, "shadow keys", torch.float32, dev)
+        s.texels = _ptr(self.texels, "shadow texels", torch.float32, dev)
+        n, w, h = self.frame.n_scenes, self.frame.width, self.frame.height
+        if tuple(self.texels.shape) != (n, w, h) or tuple(self.keys.shape) != (n, w, h):
+            raise ValueError(f"shadow keys and texels must be [{n}, {w}, {h}]")
+        s.n_tris = int(self.n_tris)
+        return s, (self.frame,)
 
 
 def binned_pairs(*, width, height, n_scenes, n_tris, tile=DEFAULT_TILE, capacity,

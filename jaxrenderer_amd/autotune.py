@@ -8,9 +8,11 @@ work-queue grid, or a grid of workgroups taking 1, 2 or 4 consecutive tiles
 each: one after another, or, at 4 pixels per lane, one wavefront per tile side
 by side), and, for each, the bin capacity sized from the binned
 triangle-tile pairs that shape produces.
-Every candidate must reproduce the first candidate's depth buffer bit for bit
-(the binning is exact, so the tile shape cannot change the image); a
-mismatch raises.
+Every candidate must reproduce the first candidate's image bit for bit (the
+binning is exact, so the tile shape cannot change it): the pass's depth
+buffer, or, tuned inside a renderer, the main pass's depth and colour; a
+mismatch raises.  With shadows on demand the main raster's cost depends on
+the shadow pass's tiles, so candidates are timed by the whole render.
 
 Results are cached in a JSON file keyed by device, pass, shader, frame size,
 scene count and the triangle count rounded to a power of two; the path is
@@ -65,7 +67,7 @@ def _store(path: str, d: Dict[str, dict]) -> None:
         pass  # a read-only home only loses the cache
 
 
-def pass_key(tag: str, call, device: torch.device) -> str:
+def pass_key(tag: str, call, device: torch.device, on_demand: bool = False) -> str:
     # the architecture and CU count identify the chip even where the marketing
     # name is unavailable (it reads empty under some profilers)
     props = torch.cuda.get_device_properties(device)
@@ -73,7 +75,7 @@ def pass_key(tag: str, call, device: torch.device) -> str:
     name = f"{arch.split(':')[0]}/{props.multi_processor_count}cu"
     tris = 1 << max(0, math.ceil(math.log2(max(call.n_tris, 1))))
     return (f"{name}|{tag}|shader{call.shader}|{call.width}x{call.height}|"
-            f"scenes{call.n_scenes}|tris<={tris}")
+            f"scenes{call.n_scenes}|tris<={tris}" + ("|shadow-on-demand" if on_demand else ""))
 
 
 def _time_launches(p, reps: int, renderer=None) -> float:
@@ -82,7 +84,7 @@ def _time_launches(p, reps: int, renderer=None) -> float:
     records on the launching stream (medians per stage keep one slow launch
     of one stage from deciding the choice).  With `renderer`, every launch is
     a whole render() -- all passes in order, as a replay runs them -- and only
-    pass `p`'s stages are counted."""
+    pass `p`'s stages are counted, or every pass's with shadows on demand."""
     import ctypes
 
     import numpy as np
@@ -106,8 +108,10 @@ def _time_launches(p, reps: int, renderer=None) -> float:
     k, P = _hip.STAGES, len(passes)
     buf = (ctypes.c_float * (k * reps * P))()
     n = L.jr_stage_times(buf, reps * P)
-    st = np.array(buf[: k * n], dtype=np.float64).reshape(n // P, P, k)[:, idx, :]
-    return float(np.median(st, axis=0).sum())
+    st = np.array(buf[: k * n], dtype=np.float64).reshape(n // P, P, k)
+    if renderer is not None and getattr(renderer, "shadow_on_demand", False):
+        return float(np.median(st, axis=0).sum())
+    return float(np.median(st[:, idx, :], axis=0).sum())
 
 
 def _apply(p, tile, form: int, ppl: int) -> None:
@@ -124,20 +128,31 @@ def tune_pass(p, candidates: Sequence = DEFAULT_CANDIDATES, reps: int = 9,
     `renderer`, inside whole renders) and leaves `p` set to the fastest.
     Returns the per-candidate timings and the choice."""
     timings: List[dict] = []
-    ref: Optional[torch.Tensor] = None
+    ref: Optional[List[torch.Tensor]] = None
+    # a pass inside a renderer is launched by whole renders: a main pass that
+    # reads the shadow pass's binning needs that binning to run first
+    run = renderer.render if renderer is not None else p.launch
     for tile, form, ppl in candidates:
         _apply(p, tile, form, ppl)
         p.set_capacity(_raster.default_capacity(p.call.n_tris, p.n_tiles))
-        p.launch()
+        run()
         torch.cuda.synchronize()
         pairs = p.pairs()
         p.set_capacity(int(pairs * SLACK) + 1024)
-        ms = _time_launches(p, reps, renderer)
-        z = p.zbuffer.view(torch.int32)
+        run()
+        torch.cuda.synchronize()
+        if renderer is not None:
+            main = renderer.passes["main"]
+            img = [main.zbuffer.view(torch.int32)]
+            if main.canvas is not None:
+                img.append(main.canvas.view(torch.int32))
+        else:
+            img = [p.zbuffer.view(torch.int32)]
         if ref is None:
-            ref = z.clone()
-        elif not torch.equal(ref, z):
-            raise RuntimeError(f"tile {tile} form {form} x{ppl} changed the depth buffer")
+            ref = [x.clone() for x in img]
+        elif not all(torch.equal(a, b) for a, b in zip(ref, img)):
+            raise RuntimeError(f"tile {tile} form {form} x{ppl} changed the image")
+        ms = _time_launches(p, reps, renderer)
         timings.append({"tile": list(tile), "form": int(form), "ppl": int(ppl), "ms": ms,
                         "pairs": int(pairs)})
     best = min(timings, key=lambda t: t["ms"])
@@ -161,7 +176,8 @@ def tune(renderer, candidates: Optional[Iterable] = None, reps: int = 9,
     chosen: Dict[str, dict] = {}
     dirty = False
     for tag, p in renderer.passes.items():
-        key = pass_key(tag, p.call, p.zbuffer.device)
+        key = pass_key(tag, p.call, p.zbuffer.device,
+                       getattr(renderer, "shadow_on_demand", False))
         hit = cache.get(key)
         entry = None if hit is None or "ppl" not in hit else (
             tuple(hit["tile"]), hit["form"], hit["ppl"])

@@ -199,7 +199,18 @@ class _Pass:
                                       n_tris=c.n_tris, tile=self.tile, capacity=self.capacity)
         self.workspace = torch.empty(need, dtype=torch.uint8, device=self.zbuffer.device)
 
-    def launch(self, part: int = _hip.PART_ALL) -> None:
+    def frame(self, part: int = _hip.PART_ALL) -> _hip.JrFrame:
+        """The jr_frame of this pass's launches."""
+        c = self.call
+        return _raster.make_frame(
+            width=c.width, height=c.height, n_scenes=c.n_scenes, shader=c.shader,
+            init_buffers=c.init_buffers, apply_offset=c.apply_offset,
+            zbuffer_default=c.zbuffer_default, colour_default=c.colour_default,
+            depth_offset=c.depth_offset, tile=self.tile, persistent=self.persistent,
+            capacity=self.capacity, tiles_per_group=self.tiles_per_group,
+            pixels_per_lane=self.pixels_per_lane, part=part)
+
+    def launch(self, part: int = _hip.PART_ALL, shadow_source=None) -> None:
         c = self.call
         _raster.forward(
             shader=c.shader, width=c.width, height=c.height, scenes=self.scenes,
@@ -210,7 +221,8 @@ class _Pass:
             apply_offset=c.apply_offset, depth_offset=c.depth_offset, tile=self.tile,
             persistent=self.persistent, tiles_per_group=self.tiles_per_group,
             pixels_per_lane=self.pixels_per_lane,
-            capacity=self.capacity, workspace=self.workspace, part=part)
+            capacity=self.capacity, workspace=self.workspace, part=part,
+            shadow_source=shadow_source)
 
     def pairs(self) -> int:
         c = self.call
@@ -219,13 +231,31 @@ class _Pass:
                                     workspace=self.workspace)
 
 
+# texels per main-pass pixel from which "auto" computes shadow texels on
+# demand: measured on config 3 (256x256 maps, 84x84 frames, 9.3 texels per
+# pixel: 1.165 -> 1.016 ms per frame) and config 5 (512x512, 480x270, 2.0:
+# 2.50 -> 2.57 ms, so off)
+ON_DEMAND_RATIO = 4
+
+
 class BatchRenderer:
     """Device-resident batched renderer over a plan from `plan_batch`."""
 
     def __init__(self, calls: List[RasterCall], *, tile=(8, 8), shadow_tile=None,
-                 persistent: bool = True, want_ids: bool = False, display: bool = False):
+                 persistent: bool = True, want_ids: bool = False, display: bool = False,
+                 shadow_on_demand=False):
         """`display`: also write the main pass's uint8 display images
-        [B, H, W, 3] (clamp to [0, 1], x 255, truncate, transpose_for_display)."""
+        [B, H, W, 3] (clamp to [0, 1], x 255, truncate, transpose_for_display).
+
+        `shadow_on_demand`: the shadow pass runs only its binning stages and
+        the main pass computes the depth of each shadow-map texel it reads
+        from that binning (jr_shadow_source), with the shadow raster's own
+        arithmetic, so the image is the same bit for bit.  The depths it
+        computed are kept in `shadow_texels` (the backward reads them there);
+        the other texels are not computed, so no whole shadow map is output.
+        "auto": on when the shadow map has at least ON_DEMAND_RATIO times as
+        many texels as the main pass has pixels (then most texels are never
+        read, and computing only the read ones is the cheaper way)."""
         dev = _raster.require_device()
         self.passes: Dict[str, _Pass] = {}
         for c in calls:
@@ -234,7 +264,19 @@ class BatchRenderer:
             self.passes[c.tag] = _Pass(c, dev, t, persistent, want_ids and main,
                                        display and main)
         main = self.passes["main"]
-        if main.call.shadow_from is not None:
+        if shadow_on_demand == "auto" and main.call.shadow_from is not None:
+            sc = self.passes[main.call.shadow_from].call
+            shadow_on_demand = (sc.width * sc.height >=
+                                ON_DEMAND_RATIO * main.call.width * main.call.height)
+        self.shadow_on_demand = bool(shadow_on_demand) and main.call.shadow_from is not None
+        self.shadow_texels: Optional[torch.Tensor] = None
+        if self.shadow_on_demand:
+            sc = self.passes[main.call.shadow_from].call
+            # zeros: texels no pixel reads are never written
+            self.shadow_texels = torch.zeros((sc.n_scenes, sc.width, sc.height),
+                                             dtype=torch.float32, device=dev)
+            main.inputs["shadow_maps"] = self.shadow_texels
+        elif main.call.shadow_from is not None:
             main.inputs["shadow_maps"] = self.passes[main.call.shadow_from].zbuffer
         for p in self.passes.values():
             # one device copy of the world positions both passes read (and a
@@ -244,9 +286,29 @@ class BatchRenderer:
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self._side: Optional[torch.cuda.Stream] = None
 
+    def _shadow_source(self) -> Optional[_raster.ShadowSource]:
+        """The main pass's view of the shadow pass's binning (on demand)."""
+        if not self.shadow_on_demand:
+            return None
+        sp = self.passes[self.passes["main"].call.shadow_from]
+        return _raster.ShadowSource(frame=sp.frame(), scenes=sp.scenes,
+                                    workspace=sp.workspace, keys=sp.zbuffer,
+                                    texels=self.shadow_texels, n_tris=sp.call.n_tris)
+
+    def _launch(self, tag: str, part: int = _hip.PART_ALL) -> None:
+        """One pass, as a render runs it: with shadows on demand the shadow
+        pass stops after binning and the main raster reads that binning."""
+        p = self.passes[tag]
+        if self.shadow_on_demand and tag != "main":
+            if part != _hip.PART_RASTER:
+                p.launch(_hip.PART_BIN)
+            return
+        src = self._shadow_source() if tag == "main" and part != _hip.PART_BIN else None
+        p.launch(part, shadow_source=src)
+
     def render(self) -> Dict[str, torch.Tensor]:
-        for p in self.passes.values():
-            p.launch()
+        for tag in self.passes:
+            self._launch(tag)
         return self._outputs()
 
     def calibrate(self, slack: float = 1.25) -> Dict[str, int]:
@@ -276,7 +338,9 @@ class BatchRenderer:
             out.update(prim_id=main.prim_id, bary=main.bary)
         if main.display is not None:
             out["display"] = main.display
-        if "shadow" in self.passes:
+        if self.shadow_on_demand:
+            out["shadow_texels"] = self.shadow_texels
+        elif "shadow" in self.passes:
             out["shadow_map"] = self.passes["shadow"].zbuffer
         return out
 
@@ -286,18 +350,17 @@ class BatchRenderer:
         for the shadow map.  Same results as render()."""
         if "shadow" not in self.passes or len(self.passes) != 2:
             return self.render()
-        shadow, main = self.passes["shadow"], self.passes["main"]
         cur = torch.cuda.current_stream()
         if self._side is None:
             self._side = torch.cuda.Stream()  # default priority
         side = self._side
         side.wait_stream(cur)
         with torch.cuda.stream(side):
-            main.launch(_hip.PART_BIN)
-        shadow.launch()
-        side.wait_stream(cur)  # the shadow map is complete
+            self._launch("main", _hip.PART_BIN)
+        self._launch("shadow")
+        side.wait_stream(cur)  # the shadow map (or its binning) is complete
         with torch.cuda.stream(side):
-            main.launch(_hip.PART_RASTER)
+            self._launch("main", _hip.PART_RASTER)
         cur.wait_stream(side)
         return self._outputs()
 

@@ -786,6 +786,121 @@ JR_DEV bool lists_tile(const Rect* __restrict__ rects, const uint32_t* __restric
   return ((m >> q) & 1u) != 0u;
 }
 
+// The shadow pass's binning as a phong_reflection_shadow raster reads it to
+// compute the shadow depths it needs itself (jr_shadow_source): the shadow
+// pass's records, tile lists and sample tables, its per-texel small-triangle
+// keys (narrow form), its frame and scene records, and the texel buffer each
+// depth found is written to.
+struct ShadowLookup {
+  const float4* rec;
+  const int32_t* counts;
+  const int32_t* offsets;
+  const int32_t* bins;
+  const Rect* rects;
+  const uint32_t* masks;
+  const float* samples;
+  const uint32_t* keys;
+  const jr_scene* scenes;
+  float* texels;
+  Frame f;
+  float zdef, depth_offset;
+};
+
+// Depth of texel round(x, y) of scene s (Shadow.get, renderer/shadow.py:127-153:
+// half away from zero, negative indices from the end, +inf out of range) as
+// the shadow pass's raster would have written it: its small-triangle key,
+// then the texel's tile list (every triangle of the scene that lists the tile
+// when the list overflowed) with the raster's edge, depth-key and tie rules,
+// the chooser's primitive-0 fallback, the default depth where nothing covers
+// the texel, plus the offset.  The depth is stored in the texel buffer.
+// Entries outside the scene's triangles are skipped, so a workspace that no
+// binning of this frame wrote gives wrong depths but no stray reads.
+JR_DEV float shadow_depth_on_demand(const ShadowLookup& L, int32_t s, float x, float y) {
+  const Frame& f = L.f;
+  int32_t ix = f2i_sat(roundf(x));
+  int32_t iy = f2i_sat(roundf(y));
+  if (ix < 0) ix += f.W;
+  if (iy < 0) iy += f.H;
+  if (ix < 0 || ix >= f.W || iy < 0 || iy >= f.H) return __builtin_inff();
+  const jr_scene& sc = L.scenes[s];
+  const int32_t b0 = sc.tri_begin, b1 = sc.tri_end;
+  const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)ix * f.H + iy;
+  const float* smp = L.samples + (int64_t)s * (f.W + f.H);
+  const float px = smp[ix], py = smp[f.W + iy];
+  const float vp22 = sc.viewport[10], vp23 = sc.viewport[11];
+  uint32_t best_key = 0xFFFFFFFFu;
+  int32_t best_id = 0x7FFFFFFF;
+  const uint32_t k0 = L.keys[pix];
+  if (k0 != ~0u) {
+    best_key = k0;
+    best_id = -1;  // small winner: its id is not kept, no equal key displaces it
+  }
+  const int32_t tx = ix >> f.tw_sh, ty = iy >> f.th_sh;
+  const int64_t gt = (int64_t)s * f.T + (int64_t)tx * f.tiles_y + ty;
+  const int32_t cnt = L.counts[gt];
+  const int64_t start = L.offsets[gt];
+  const bool overflow = start < 0 || cnt < 0 || start + cnt > f.capacity;
+  const int32_t n = overflow ? b1 - b0 : cnt;
+  for (int32_t m = 0; m < n; ++m) {
+    const int32_t t = overflow ? b0 + m : L.bins[start + m];
+    if (t < b0 || t >= b1) continue;
+    if (overflow && !lists_tile(L.rects, L.masks, t, tx, ty)) continue;
+    const float4 r0 = L.rec[4 * (int64_t)t], r1 = L.rec[4 * (int64_t)t + 1],
+                 r2 = L.rec[4 * (int64_t)t + 2];
+    const float e0 = edge(r0, px, py), e1 = edge(r1, px, py), e2 = edge(r2, px, py);
+    if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
+      const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
+      if (key < best_key || (key == best_key && t < best_id)) {
+        best_key = key;
+        best_id = t;
+      }
+    }
+  }
+  float d;
+  if (best_id == -1 ||
+      (best_key < kInfKey && best_key != 0u && best_key != 0x80000000u)) {
+    d = key_depth(best_key) + L.depth_offset;
+  } else {
+    // no finite winner (argmin's index 0 decides with its own keep & inside),
+    // or a NaN / zero key whose depth the key does not give back
+    int32_t w = -1;
+    bool cover = false;
+    if (b1 > b0) {
+      if (best_key >= kInfKey) {
+        w = b0;
+        const float4 r0 = L.rec[4 * (int64_t)w], r1 = L.rec[4 * (int64_t)w + 1],
+                     r2 = L.rec[4 * (int64_t)w + 2];
+        const uint32_t fl = __float_as_uint(L.rec[4 * (int64_t)w + 3].y);
+        cover = (fl & kKeep) && edge(r0, px, py) >= 0.0f && edge(r1, px, py) >= 0.0f &&
+                edge(r2, px, py) >= 0.0f;
+      } else {
+        w = best_id;
+        cover = true;
+      }
+    }
+    if (cover) {
+      const float4 r0 = L.rec[4 * (int64_t)w], r1 = L.rec[4 * (int64_t)w + 1],
+                   r2 = L.rec[4 * (int64_t)w + 2];
+      const float e0 = edge(r0, px, py), e1 = edge(r1, px, py), e2 = edge(r2, px, py);
+      d = win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23) + L.depth_offset;
+    } else {
+      d = L.zdef + L.depth_offset;
+    }
+  }
+  L.texels[pix] = d;
+  return d;
+}
+
+// The shadow shader's depth source in the raster: the shadow pass's binning
+// when the call has a shadow source, else the finished map.
+struct RasterShadowDepth {
+  const ShadowLookup* L;
+  JR_DEV float operator()(const jr_inputs& in, int scene, float x, float y) const {
+    if (L) return shadow_depth_on_demand(*L, scene, x, y);
+    return MapShadowDepth()(in, scene, x, y);
+  }
+};
+
 // Stage 4: per-tile visibility, shading and merge.
 struct RasterArgs {
   jr_inputs in;
@@ -807,6 +922,8 @@ struct RasterArgs {
   int32_t pixels_per_lane;  // 1, or 4 (16x16 and 8x32 tiles): pixels per thread
   int32_t init_buffers, apply_offset;
   float zdef, cdef0, cdef1, cdef2, depth_offset;
+  int32_t shadow_on_demand;  // phong_reflection_shadow: depths from `sl`
+  ShadowLookup sl;
 };
 
 // Chooser fallback, shading, merge and stores of pixel (i, j) of scene s
@@ -1003,7 +1120,12 @@ JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t 
       } else {
         V::corners(a.in, sc, corner_slots(a.in, h[q].w), v);
       }
-      keep = V::pixel(a.in, sc, v, l, h[q].b, (h[q].fl & kFront) != 0, col, s);
+      if constexpr (SHADER == JR_SHADER_PHONG_REFLECTION_SHADOW) {
+        const RasterShadowDepth depth{a.shadow_on_demand ? &a.sl : nullptr};
+        keep = V::pixel(a.in, sc, v, l, h[q].b, (h[q].fl & kFront) != 0, col, s, depth);
+      } else {
+        keep = V::pixel(a.in, sc, v, l, h[q].b, (h[q].fl & kFront) != 0, col, s);
+      }
     }
     store_pixel(a, s, pi[q], pj[q], h[q], keep, col);
   }
@@ -1278,6 +1400,37 @@ int32_t log2_exact(int32_t v) {
   return l;
 }
 
+// The kernels' view of a frame; `narrow`: the depth-only form whose
+// small-triangle keys stay in the zbuffer.
+Frame dev_frame(const jr_frame* fr, bool narrow) {
+  Frame f;
+  f.W = fr->width;
+  f.H = fr->height;
+  f.TW = fr->tile_w;
+  f.TH = fr->tile_h;
+  f.tiles_x = (f.W + f.TW - 1) / f.TW;
+  f.tiles_y = (f.H + f.TH - 1) / f.TH;
+  f.T = f.tiles_x * f.tiles_y;
+  f.tw_sh = log2_exact(f.TW);
+  f.th_sh = log2_exact(f.TH);
+  magic31((uint32_t)f.T, &f.T_m, &f.T_sh);
+  magic31((uint32_t)f.tiles_y, &f.ty_m, &f.ty_sh);
+  f.n_scenes = fr->n_scenes;
+  f.capacity = fr->bin_capacity;
+  f.narrow = narrow;
+  return f;
+}
+
+// A depth pass with an offset and no id outputs keeps 32-bit depth keys in
+// its zbuffer (an offset of -0 would keep the sign of a -0 depth, which a key
+// cannot tell apart).
+bool narrow_form(const jr_frame* fr, bool ids) {
+  uint32_t ob;
+  std::memcpy(&ob, &fr->depth_offset, 4);
+  return fr->shader == JR_SHADER_DEPTH && !ids && fr->apply_offset && fr->init_buffers &&
+         ob != 0x80000000u;
+}
+
 WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   WsLayout L{};
   const int64_t tiles_x = (fr->width + fr->tile_w - 1) / fr->tile_w;
@@ -1402,6 +1555,28 @@ void launch_raster(int shader, const RasterArgs& a, int64_t n_items,
   }
 }
 
+// A shadow source must describe a depth pass in the narrow form, of this
+// call's scenes and shadow-map size, whose workspace holds its binning.
+int32_t check_shadow_source(const jr_frame* fr, const jr_inputs* in) {
+  const jr_shadow_source* src = in->shadow_source;
+  if (fr->shader != JR_SHADER_PHONG_REFLECTION_SHADOW)
+    return fail(JR_E_INVALID, "shadow_source is read only by phong_reflection_shadow");
+  if (!src->frame || !src->scenes || !src->workspace || !src->keys || !src->texels)
+    return fail(JR_E_INVALID, "shadow_source: frame, scenes, workspace, keys and texels "
+                              "are required");
+  const jr_frame* sf = src->frame;
+  int32_t rc = check_frame(sf);
+  if (rc) return rc;
+  if (!narrow_form(sf, false))
+    return fail(JR_E_INVALID, "shadow_source: the shadow pass must be a depth pass with "
+                              "init_buffers, apply_offset and an offset other than -0");
+  if (sf->n_scenes != fr->n_scenes || sf->width != in->shadow_w || sf->height != in->shadow_h)
+    return fail(JR_E_INVALID, "shadow_source: scene count or map size differs from this call");
+  if (src->n_tris < 0 || src->workspace_bytes < ws_layout(sf, src->n_tris).total)
+    return fail(JR_E_WORKSPACE, "shadow_source: workspace smaller than its frame needs");
+  return JR_OK;
+}
+
 int32_t check_inputs(const jr_frame* fr, const jr_inputs* in, const jr_outputs* out) {
   if (!in || !out) return fail(JR_E_INVALID, "inputs/outputs NULL");
   if (in->n_tris < 0 || in->n_slots < 0) return fail(JR_E_INVALID, "negative sizes");
@@ -1429,9 +1604,9 @@ int32_t check_inputs(const jr_frame* fr, const jr_inputs* in, const jr_outputs* 
   if (sh == JR_SHADER_PHONG_DARBOUX && (!in->id_to_face || !in->face_slots || !in->normal_maps))
     return fail(JR_E_INVALID, "phong_darboux needs id_to_face, face_slots, normal_maps");
   if (sh == JR_SHADER_PHONG_REFLECTION_SHADOW &&
-      (!in->shadow_maps || in->shadow_w <= 0 || in->shadow_h <= 0))
+      ((!in->shadow_maps && !in->shadow_source) || in->shadow_w <= 0 || in->shadow_h <= 0))
     return fail(JR_E_INVALID, "phong_reflection_shadow needs shadow maps");
-  return JR_OK;
+  return in->shadow_source ? check_shadow_source(fr, in) : JR_OK;
 }
 
 }  // namespace jr
@@ -1452,6 +1627,7 @@ int64_t jr_sizeof(int32_t which) {
     case 3: return sizeof(jr_frame);
     case 4: return sizeof(jr_grads);
     case 5: return sizeof(jr_pose_args);
+    case 6: return sizeof(jr_shadow_source);
     default: return -1;
   }
 }
@@ -1472,27 +1648,7 @@ namespace jr {
 int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
                      jr_outputs* out, char* ws, const WsLayout& L, hipStream_t st,
                      bool timed) {
-  Frame f;
-  f.W = fr->width;
-  f.H = fr->height;
-  f.TW = fr->tile_w;
-  f.TH = fr->tile_h;
-  f.tiles_x = (f.W + f.TW - 1) / f.TW;
-  f.tiles_y = (f.H + f.TH - 1) / f.TH;
-  f.T = f.tiles_x * f.tiles_y;
-  f.tw_sh = log2_exact(f.TW);
-  f.th_sh = log2_exact(f.TH);
-  magic31((uint32_t)f.T, &f.T_m, &f.T_sh);
-  magic31((uint32_t)f.tiles_y, &f.ty_m, &f.ty_sh);
-  f.n_scenes = fr->n_scenes;
-  f.capacity = fr->bin_capacity;
-  {
-    uint32_t ob;
-    std::memcpy(&ob, &fr->depth_offset, 4);
-    // -0 would keep the sign of a -0 depth, which a key cannot tell apart
-    f.narrow = fr->shader == JR_SHADER_DEPTH && !out->prim_id && !out->bary &&
-               fr->apply_offset && fr->init_buffers && ob != 0x80000000u;
-  }
+  const Frame f = dev_frame(fr, narrow_form(fr, out->prim_id || out->bary));
 
   float4* rec = reinterpret_cast<float4*>(ws + L.rec);
   Rect* rects = reinterpret_cast<Rect*>(ws + L.rects);
@@ -1511,7 +1667,8 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   uint32_t* wbits = reinterpret_cast<uint32_t*>(ws + L.wbits);
   const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
 
-  StageTimer timer(st, timed && fr->part == JR_PART_ALL);
+  // a binning-only call is timed too (its raster stage then takes no time)
+  StageTimer timer(st, timed && fr->part != JR_PART_RASTER);
   const int n_tris = in->n_tris;
   if (fr->part != JR_PART_RASTER) {
   // counts, cursor and misc are contiguous: clear them, and the key buffer,
@@ -1552,6 +1709,7 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   timer.mark(5);
   }
   if (fr->part == JR_PART_BIN) {
+    timer.mark(6);
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess)
       return fail(JR_E_HIP, std::string("launch failed: ") + hipGetErrorString(err));
@@ -1582,6 +1740,27 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   a.cdef1 = fr->colour_default[1];
   a.cdef2 = fr->colour_default[2];
   a.depth_offset = fr->depth_offset;
+  a.shadow_on_demand = 0;
+  if (in->shadow_source) {
+    // validated by check_shadow_source
+    const jr_shadow_source& src = *in->shadow_source;
+    const WsLayout SL = ws_layout(src.frame, src.n_tris);
+    const char* sw = static_cast<const char*>(src.workspace);
+    a.sl.rec = reinterpret_cast<const float4*>(sw + SL.rec);
+    a.sl.counts = reinterpret_cast<const int32_t*>(sw + SL.counts);
+    a.sl.offsets = reinterpret_cast<const int32_t*>(sw + SL.offsets);
+    a.sl.bins = reinterpret_cast<const int32_t*>(sw + SL.bins);
+    a.sl.rects = reinterpret_cast<const Rect*>(sw + SL.rects);
+    a.sl.masks = reinterpret_cast<const uint32_t*>(sw + SL.masks);
+    a.sl.samples = reinterpret_cast<const float*>(sw + SL.samples);
+    a.sl.keys = reinterpret_cast<const uint32_t*>(src.keys);
+    a.sl.scenes = src.scenes;
+    a.sl.texels = src.texels;
+    a.sl.f = dev_frame(src.frame, true);
+    a.sl.zdef = src.frame->zbuffer_default;
+    a.sl.depth_offset = src.frame->depth_offset;
+    a.shadow_on_demand = 1;
+  }
   if (n_tris == 0) {
     // no primitive: every pixel keeps its old value (plus the offset)
     if (!fr->init_buffers && !fr->apply_offset) {

@@ -95,6 +95,16 @@ JR_DEV float shadow_get(const float* __restrict__ map, int32_t sw, int32_t sh,
   return map[(int64_t)ix * sh + iy];
 }
 
+// Depth of the shadow map at light-viewport position (x, y) of a scene, read
+// from a finished map (in.shadow_maps).  A raster may hand the shadow shader
+// another source of the same depths.
+struct MapShadowDepth {
+  JR_DEV float operator()(const jr_inputs& in, int scene, float x, float y) const {
+    const float* map = in.shadow_maps + (int64_t)scene * in.shadow_w * in.shadow_h;
+    return shadow_get(map, in.shadow_w, in.shadow_h, x, y);
+  }
+};
+
 // Phong reflection fragment core shared by the two reflection shaders
 // (shaders/phong_reflection.py:154-235, phong_reflection_shadow.py:173-271).
 // Computes tex colour, diffuse scalar and specular scalar.
@@ -436,9 +446,10 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
   static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
     normalise3(sc.light_dir_eye, l);
   }
+  template <class ShadowDepth = MapShadowDepth>
   static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
                            const float l[3], const float b[3], bool front, float col[3],
-                           int scene) {
+                           int scene, const ShadowDepth& shadow_depth = ShadowDepth()) {
     const float* ne = v;
     const float* uvv = v + 9;
     const float* scd = v + 15;
@@ -454,8 +465,7 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
     float sv[4];
     apply4(sc.shadow_viewport, s4[0], s4[1], s4[2], s4[3], sv);
     const float sx = sv[0] / sv[3], sy = sv[1] / sv[3], sz = sv[2] / sv[3];
-    const float* map = in.shadow_maps + (int64_t)scene * in.shadow_w * in.shadow_h;
-    const float sd = shadow_get(map, in.shadow_w, in.shadow_h, sx, sy);
+    const float sd = shadow_depth(in, scene, sx, sy);
     const bool lit = sz <= sd;
     float tex[3], diff, spec;
     reflection_terms(in, sc, n, uv, ti, __float_as_int(v[kShape]), __float_as_int(v[kShape + 1]), l,
