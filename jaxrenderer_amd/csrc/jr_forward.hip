@@ -476,8 +476,10 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
                                                    samples + (int64_t)s * (f.W + f.H), f));
     // the record is read for binned triangles, for pixel winners and for the
     // scene's first triangle; the narrow form finishes its small winners
-    // from their keys alone
-    if (!(f.narrow && small) || t == sc.tri_begin) {
+    // from their keys alone, and a triangle whose pixel range is empty is
+    // never binned (its mask is empty, so not even an overflowing tile's scan
+    // reads it) and never wins
+    if ((area > 0 && !(f.narrow && small)) || t == sc.tri_begin) {
       rec[4 * (int64_t)t + 0] = r0;
       rec[4 * (int64_t)t + 1] = r1;
       rec[4 * (int64_t)t + 2] = r2;

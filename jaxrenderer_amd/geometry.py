@@ -59,6 +59,13 @@ def normalise(vector: ArrayLike):
     if _is_torch(vector):
         import torch
 
+        if vector.numel() <= 4:
+            # small vectors: the sum of squares left to right, as the kernels
+            flat = vector.reshape(-1)
+            ss = flat[0] * flat[0]
+            for k in range(1, flat.shape[0]):
+                ss = ss + flat[k] * flat[k]
+            return vector / torch.sqrt(ss)
         return vector / torch.linalg.norm(vector)
     v = asf32(vector)
     n = f32(np.sqrt(_sum_sq(v)))
@@ -237,8 +244,17 @@ class Camera(NamedTuple):
     def apply(points: ArrayLike, matrix: ArrayLike) -> np.ndarray:
         """out_i = sum_j p_j M[i, j] (renderer/geometry.py:280-315)."""
         if _is_torch(points):
+            import torch
+
             M = _torch_matrix(matrix, points)
-            return points @ M.transpose(0, 1)
+            # sums left to right, one rounding per operation, as the kernels
+            rows = []
+            for i in range(M.shape[0]):
+                s = points[..., 0] * M[i, 0]
+                for j in range(1, points.shape[-1]):
+                    s = s + points[..., j] * M[i, j]
+                rows.append(s)
+            return torch.stack(rows, dim=-1)
         p = asf32(points)
         M = asf32(matrix)
         single = p.ndim == 1

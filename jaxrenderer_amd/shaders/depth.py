@@ -2,7 +2,8 @@
 from typing import Any, NamedTuple
 
 from .._hip import SHADER_DEPTH
-from ..shader import Shader
+from ..shader import PerVertex, Shader
+from . import _ops
 
 
 class DepthExtraInput(NamedTuple):
@@ -18,6 +19,13 @@ class DepthExtraMixerOutput(NamedTuple):
 
 
 class DepthShader(Shader):
-    """Depth only (used by the shadow-map pass)."""
+    """Depth only (used by the shadow-map pass).  The default interpolate,
+    fragment and mix hooks; the class itself runs as the fused depth kernel."""
 
     _jr_shader_id = SHADER_DEPTH
+
+    @staticmethod
+    def vertex(gl_VertexID, gl_InstanceID, camera, extra):
+        """renderer/shaders/depth.py:43-61: gl_Position = camera.to_clip(position)."""
+        position = _ops.take(extra.position, gl_VertexID)
+        return PerVertex(gl_Position=_ops.to_clip(camera, position)), DepthExtraFragmentData()
