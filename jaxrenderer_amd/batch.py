@@ -36,6 +36,9 @@ class BatchScene(NamedTuple):
     model: MergedModel
     camera: Camera
     light: LightParameters
+    # ShadowParameters.centre of this scene (renderer/renderer.py:120-130);
+    # None: the batch's ShadowMapSpec.centre
+    shadow_centre: Optional[Tuple[float, float, float]] = None
 
 
 class ShadowMapSpec(NamedTuple):
@@ -113,7 +116,8 @@ def plan_batch(scenes: Sequence[BatchScene], width: int, height: int,
             tex_offset=int(m.offset),
         )
         if shadow is not None:
-            scam = Shadow.camera_for(light.direction, svp, shadow.centre, shadow.up,
+            centre = shadow.centre if s.shadow_centre is None else s.shadow_centre
+            scam = Shadow.camera_for(light.direction, svp, centre, shadow.up,
                                      shadow.distance)
             sc.update(shadow_world_to_clip=scam.world_to_clip, shadow_viewport=scam.viewport,
                       shadow_strength=asf32(shadow.strength))

@@ -893,15 +893,21 @@ JR_DEV float shadow_depth_on_demand(const ShadowLookup& L, int32_t s, float x, f
   return d;
 }
 
-// The shadow shader's depth source in the raster: the shadow pass's binning
-// when the call has a shadow source, else the finished map.
-struct RasterShadowDepth {
+// The shadow shader's depth source in the raster of a call with a shadow
+// source: the shadow pass's binning.
+struct LookupShadowDepth {
   const ShadowLookup* L;
-  JR_DEV float operator()(const jr_inputs& in, int scene, float x, float y) const {
-    if (L) return shadow_depth_on_demand(*L, scene, x, y);
-    return MapShadowDepth()(in, scene, x, y);
+  JR_DEV float operator()(const jr_inputs&, int scene, float x, float y) const {
+    return shadow_depth_on_demand(*L, scene, x, y);
   }
 };
+
+// Raster instance of phong_reflection_shadow for calls with a shadow source:
+// its own template instance, so the instance that reads a finished map does
+// not carry the lookup's code and registers.
+constexpr int kShaderShadowOnDemand = JR_SHADER_COUNT;
+template <>
+struct Varyings<kShaderShadowOnDemand> : Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {};
 
 // Stage 4: per-tile visibility, shading and merge.
 struct RasterArgs {
@@ -1122,8 +1128,8 @@ JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t 
       } else {
         V::corners(a.in, sc, corner_slots(a.in, h[q].w), v);
       }
-      if constexpr (SHADER == JR_SHADER_PHONG_REFLECTION_SHADOW) {
-        const RasterShadowDepth depth{a.shadow_on_demand ? &a.sl : nullptr};
+      if constexpr (SHADER == kShaderShadowOnDemand) {
+        const LookupShadowDepth depth{&a.sl};
         keep = V::pixel(a.in, sc, v, l, h[q].b, (h[q].fl & kFront) != 0, col, s, depth);
       } else {
         keep = V::pixel(a.in, sc, v, l, h[q].b, (h[q].fl & kFront) != 0, col, s);
@@ -1552,7 +1558,12 @@ void launch_raster(int shader, const RasterArgs& a, int64_t n_items,
     case JR_SHADER_PHONG: launch_raster_s<JR_SHADER_PHONG>(a, n_items, persistent, st); break;
     case JR_SHADER_PHONG_DARBOUX: launch_raster_s<JR_SHADER_PHONG_DARBOUX>(a, n_items, persistent, st); break;
     case JR_SHADER_PHONG_REFLECTION: launch_raster_s<JR_SHADER_PHONG_REFLECTION>(a, n_items, persistent, st); break;
-    case JR_SHADER_PHONG_REFLECTION_SHADOW: launch_raster_s<JR_SHADER_PHONG_REFLECTION_SHADOW>(a, n_items, persistent, st); break;
+    case JR_SHADER_PHONG_REFLECTION_SHADOW:
+      if (a.shadow_on_demand)
+        launch_raster_s<kShaderShadowOnDemand>(a, n_items, persistent, st);
+      else
+        launch_raster_s<JR_SHADER_PHONG_REFLECTION_SHADOW>(a, n_items, persistent, st);
+      break;
     default: launch_raster_s<JR_SHADER_VISIBILITY>(a, n_items, persistent, st); break;
   }
 }

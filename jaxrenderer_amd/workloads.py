@@ -256,9 +256,91 @@ def config4(seed: int = 0, n_views: int = 64, size: int = 256, tex_size: int = 5
     return cams, f, v, n, uv, colours, tex
 
 
+def _quat_z_to(d: np.ndarray) -> np.ndarray:
+    """Unit quaternion (w, x, y, z) rotating +z onto the unit vector d."""
+    z = np.array((0.0, 0.0, 1.0))
+    c = float(np.dot(z, d))
+    if c < -1.0 + 1e-9:
+        return np.array((0.0, 1.0, 0.0, 0.0), f32)
+    axis = np.cross(z, d)
+    q = np.array((1.0 + c, axis[0], axis[1], axis[2]))
+    return (q / np.linalg.norm(q)).astype(f32)
+
+
+def ant_like_objects(rng, torso_xy=(0.0, 0.0)) -> Tuple[List[ModelObject], np.ndarray]:
+    """One frame of an ant-like walker, the composition of the reference's
+    Brax ant benchmark scene (notebooks/84x84/A100.ipynb; 18 objects, 3,276
+    triangles): a ground box (12 triangles) and 17 create_capsule capsules
+    (192 triangles each) -- a spherical torso and four legs of four segments
+    (hip, upper, thigh, shin) at joint angles drawn from `rng`.  Returns the
+    objects and the torso centre."""
+    from .shapes.capsule import UpAxis, create_capsule
+
+    spec8 = np.full((8, 8), 2.0, f32)
+    ground = create_cube(np.array((10.0, 10.0, 0.05), f32), np.full(2, 4.0, f32),
+                         rng.random((8, 8, 3)).astype(f32), spec8)
+    gt = np.identity(4, dtype=f32)
+    gt[2, 3] = -0.05
+    objs = [ModelObject(model=ground, transform=gt)]
+    torso = np.array((torso_xy[0], torso_xy[1], 0.75 + 0.1 * rng.uniform(-1, 1)), f32)
+    tex = rng.random((8, 8, 3)).astype(f32)
+
+    def capsule(a: np.ndarray, b: np.ndarray, radius: float) -> ModelObject:
+        d = b - a
+        half = float(np.linalg.norm(d)) / 2.0
+        t = np.identity(4, dtype=f32)
+        t[:3, :3] = transform_matrix_from_rotation(_quat_z_to(d / (2.0 * half)))
+        t[:3, 3] = ((a + b) / 2.0).astype(f32)
+        model = create_capsule(f32(radius), f32(max(half - radius, 0.0)), UpAxis.Z, tex, spec8)
+        return ModelObject(model=model, transform=t)
+
+    sphere = create_capsule(f32(0.25), f32(0.0), UpAxis.Z, tex, spec8)
+    tt = np.identity(4, dtype=f32)
+    tt[:3, 3] = torso
+    objs.append(ModelObject(model=sphere, transform=tt))
+    yaw = rng.uniform(0.0, 2.0 * math.pi)
+    for k in range(4):
+        a = yaw + math.pi / 4 + k * math.pi / 2 + 0.3 * rng.uniform(-1, 1)
+        out = np.array((math.cos(a), math.sin(a), 0.0))
+        p0 = torso + 0.2 * out
+        p1 = p0 + 0.15 * out
+        p2 = p1 + 0.2 * out + np.array((0.0, 0.0, 0.05 * rng.uniform(-1, 1)))
+        lift = rng.uniform(0.2, 0.6)
+        p3 = p2 + 0.28 * (math.cos(lift) * out + np.array((0.0, 0.0, -math.sin(lift))))
+        p4 = np.array((p3[0] + 0.15 * out[0], p3[1] + 0.15 * out[1], 0.08))
+        for s0, s1 in ((p0, p1), (p1, p2), (p2, p3), (p3, p4)):
+            objs.append(capsule(s0.astype(f32), s1.astype(f32), 0.08))
+    return objs, torso
+
+
+def ant_like(n_scenes: int = 200, seed: int = 0, width: int = 84, height: int = 84
+             ) -> Tuple[List[BatchScene], ShadowMapSpec]:
+    """Frames of ant-like walkers rendered as the reference's Brax benchmark
+    renders its ant (notebooks/84x84/A100.ipynb cells 4 and 7): 3,276
+    triangles per frame, a camera tracking the torso, the notebook's light,
+    a shadow map the size of the canvas centred on the camera target
+    (get_camera_image with ShadowParameters(centre=target)),
+    phong_reflection_shadow."""
+    rng = _rng(seed)
+    scenes = []
+    for i in range(n_scenes):
+        xy = rng.uniform(-2.0, 2.0, size=2)
+        objs, torso = ant_like_objects(rng, tuple(xy))
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        target = tuple(float(v) for v in torso)
+        cp = CameraParameters(viewWidth=width, viewHeight=height, hfov=60.0, vfov=60.0,
+                              position=(torso[0] + 2.5 * math.cos(az),
+                                        torso[1] + 2.5 * math.sin(az), 1.8),
+                              target=target, up=(0.0, 0.0, 1.0))
+        scenes.append(BatchScene(model=merge_objects(objs),
+                                 camera=Renderer.create_camera_from_parameters(cp),
+                                 light=NOTEBOOK_LIGHT, shadow_centre=target))
+    return scenes, ShadowMapSpec(width=width, height=height)
+
+
 def triangle_counts(scenes: Sequence[BatchScene]) -> np.ndarray:
     return np.array([len(s.model.faces) for s in scenes])
 
 
-__all__ = ["config1", "config2", "config3", "config4", "config5", "brax_like_objects",
+__all__ = ["ant_like", "ant_like_objects", "config1", "config2", "config3", "config4", "config5", "brax_like_objects",
            "dense_brax_like_objects", "uv_sphere", "NOTEBOOK_LIGHT"]

@@ -220,3 +220,18 @@ def test_config5_range_ends_480x270():
                          pixels_per_lane=4)
     pid = as_np(got["main"]["prim_id"])
     assert (pid[0] >= 0).sum() > 1000 and (pid[1] >= 0).sum() > 1000
+
+
+def test_ant_like_frames_against_oracle():
+    """The ant-like benchmark frames (18 objects, 3,276 triangles, shadow map
+    the size of the 84x84 canvas centred on each camera's target) batched,
+    against the oracle."""
+    scs, sm = workloads.ant_like(n_scenes=3, seed=7)
+    assert set(workloads.triangle_counts(scs)) == {3276}
+    assert all(len(s.model.texture_shape) == 18 for s in scs)
+    assert (sm.width, sm.height) == (84, 84)
+    got, _ = _check_plan(lambda: batch.plan_batch(scs, 84, 84, sm), "ant")
+    pid = as_np(got["main"]["prim_id"])
+    for s in range(3):
+        # the walker itself (faces after the ground box's 12) is on screen
+        assert ((pid[s] >= s * 3276 + 12) & (pid[s] < (s + 1) * 3276)).sum() > 200
