@@ -194,9 +194,23 @@ def main() -> None:
     scenes, smspec = workloads.config3(n_scenes=B, seed=args.seed, width=W, height=H,
                                        objects=objects)
     tris = workloads.triangle_counts(scenes)
-    calls = batch.plan_batch(scenes, W, H, smspec)
     log(f"generated {B} scenes ({tris.sum()} triangles, {tris.min()}..{tris.max()} per scene) "
         f"in {time.time() - t0:.1f}s")
+    # planning the batch: the per-scene host planner, and the device planner
+    # (jr_pack plus batched records) whose calls are rendered below; both
+    # give the same calls bit for bit
+    from jaxrenderer_amd import packing
+
+    t0 = time.perf_counter()
+    batch.plan_batch(scenes, W, H, smspec)
+    plan_host_ms = (time.perf_counter() - t0) * 1e3
+    packing.plan_batch_device(scenes[:8], W, H, smspec)  # first-call set-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    calls = packing.plan_batch_device(scenes, W, H, smspec)
+    torch.cuda.synchronize()
+    plan_dev_ms = (time.perf_counter() - t0) * 1e3
+    log(f"planning {B} scenes: host {plan_host_ms:.0f} ms, device {plan_dev_ms:.1f} ms")
 
     def wh(s):
         w, h = (int(v) for v in s.lower().split("x"))
@@ -498,6 +512,8 @@ def main() -> None:
         "fwd_bwd_grads_finite": grads_finite,
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "pose_update": pose,
+        "planning_ms": {"host_per_scene": round(plan_host_ms, 1),
+                        "device": round(plan_dev_ms, 2)},
         "binned_pairs": pairs,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -236,7 +236,7 @@ const char* jr_last_error(void);
 
 /* sizeof of the ABI structs, for binding-side layout checks:
  * 0 jr_scene, 1 jr_inputs, 2 jr_outputs, 3 jr_frame, 4 jr_grads,
- * 5 jr_pose_args, 6 jr_shadow_source. */
+ * 5 jr_pose_args, 6 jr_shadow_source, 7 jr_pack_args. */
 int64_t jr_sizeof(int32_t which);
 
 /* Bytes of workspace jr_forward needs for this frame (bin_capacity included).
@@ -357,6 +357,41 @@ typedef struct jr_pose_args {
   float* normals;             /* [n_norms, 3], written; NULL: skipped */
 } jr_pose_args;
 int32_t jr_pose(const jr_pose_args* args, jr_stream_t stream);
+
+/* The per-corner flattening of a batch (renderer/renderer.py:277-296,
+ * Renderer.render's position = verts[faces.ravel()] and likewise normals, uvs
+ * and texture_index[faces_uv.ravel()]) for scenes whose rows are concatenated:
+ * writes the global index arrays jr_inputs.pos_index / nrm_index / uv_index /
+ * tex_index from each scene's local face tables and the offsets of its rows.
+ * Replaces the index arithmetic a batch planner does per scene on the host.
+ * The *_off arrays hold n_scenes + 1 ascending int64 offsets (first 0): scene
+ * s owns face rows [tri_off[s], tri_off[s + 1]) (3 corners each, in faces,
+ * faces_norm, faces_uv), vertex rows [vert_off[s], vert_off[s + 1]), and so on.
+ * A local index outside its scene's rows sets *error to 1 (the caller
+ * synchronises and reports it) and is written clamped into the scene's rows.
+ * tex_index may be NULL (then texidx_off and texture_index are not read). */
+typedef struct jr_pack_args {
+  int32_t n_scenes;
+  int32_t pad;
+  int64_t n_faces;            /* total face rows */
+  const int64_t* tri_off;     /* [n_scenes + 1] */
+  const int64_t* vert_off;    /* [n_scenes + 1] */
+  const int64_t* norm_off;    /* [n_scenes + 1] */
+  const int64_t* uv_off;      /* [n_scenes + 1] */
+  const int64_t* texidx_off;  /* [n_scenes + 1] rows of texture_index */
+  const int32_t* faces;       /* [3 n_faces] local vertex indices */
+  const int32_t* faces_norm;  /* [3 n_faces] local normal indices */
+  const int32_t* faces_uv;    /* [3 n_faces] local uv indices */
+  const int32_t* texture_index; /* per-vertex texture map index, concatenated */
+  int32_t* pos_index;         /* [3 n_faces], written */
+  int32_t* nrm_index;         /* [3 n_faces], written */
+  int32_t* uv_index;          /* [3 n_faces], written */
+  int32_t* tex_index;         /* [3 n_faces], written; optional */
+  int32_t* error;             /* one word, OR-ed with 1 on a bad index */
+} jr_pack_args;
+int32_t jr_pack(const jr_pack_args* args, jr_stream_t stream);
+/* Message of the calling thread's last failed jr_pack. */
+const char* jr_pack_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,29 @@ class JrShadowSource(ctypes.Structure):
     ]
 
 
+class JrPackArgs(ctypes.Structure):
+    """jr_pack_args: the per-corner flattening of a batch on the device."""
+    _fields_ = [
+        ("n_scenes", c_i32),
+        ("pad", c_i32),
+        ("n_faces", c_i64),
+        ("tri_off", c_vp),
+        ("vert_off", c_vp),
+        ("norm_off", c_vp),
+        ("uv_off", c_vp),
+        ("texidx_off", c_vp),
+        ("faces", c_vp),
+        ("faces_norm", c_vp),
+        ("faces_uv", c_vp),
+        ("texture_index", c_vp),
+        ("pos_index", c_vp),
+        ("nrm_index", c_vp),
+        ("uv_index", c_vp),
+        ("tex_index", c_vp),
+        ("error", c_vp),
+    ]
+
+
 class JrGrads(ctypes.Structure):
     _fields_ = [
         ("d_positions", c_vp),
@@ -197,7 +220,8 @@ DSCENE = {
     "shadow_strength": 66,
 }
 
-STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads, JrPoseArgs, JrShadowSource)
+STRUCTS = (JrScene, JrInputs, JrOutputs, JrFrame, JrGrads, JrPoseArgs, JrShadowSource,
+           JrPackArgs)
 
 
 class HipError(RuntimeError):
@@ -231,6 +255,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.jr_backward_last_error.restype = ctypes.c_char_p
     lib.jr_pose.argtypes = [P(JrPoseArgs), c_vp]
     lib.jr_pose.restype = c_i32
+    lib.jr_pack.argtypes = [P(JrPackArgs), c_vp]
+    lib.jr_pack.restype = c_i32
+    lib.jr_pack_last_error.restype = ctypes.c_char_p
 
 
 def check_layout(lib: ctypes.CDLL) -> None:
@@ -266,6 +293,7 @@ def lib() -> ctypes.CDLL:
 
 def check(rc: int, what: str) -> None:
     if rc != 0:
-        getter = lib().jr_backward_last_error if what == "jr_backward" else lib().jr_last_error
+        getter = {"jr_backward": lib().jr_backward_last_error,
+                  "jr_pack": lib().jr_pack_last_error}.get(what, lib().jr_last_error)
         msg = getter().decode(errors="replace")
         raise HipError(f"{what} failed with code {rc}: {msg}")

@@ -1641,6 +1641,7 @@ int64_t jr_sizeof(int32_t which) {
     case 4: return sizeof(jr_grads);
     case 5: return sizeof(jr_pose_args);
     case 6: return sizeof(jr_shadow_source);
+    case 7: return sizeof(jr_pack_args);
     default: return -1;
   }
 }

@@ -20,3 +20,21 @@ def test_flattened_main_call_has_identity_faces():
     dev = _plan.device_inputs(main, torch.device("cpu"))
     assert dev["faces"] is None and dev["positions"] is not None
     assert _plan.device_inputs(shadow, torch.device("cpu"))["faces"] is not None
+
+
+def test_batched_scene_records_equal_the_per_scene_planner():
+    """packing.scene_records (all scenes at once in numpy) gives the bytes
+    plan_batch's per-scene code puts in its calls, for the main and the
+    shadow pass, with batch-wide and per-scene shadow centres."""
+    from jaxrenderer_amd import batch, packing, workloads
+
+    for scs, sm, W, H in ((*workloads.config3(n_scenes=40, seed=3), 84, 84),
+                          (*workloads.ant_like(n_scenes=6, seed=2), 84, 84),
+                          ([workloads.config1()], None, 320, 240)):
+        host = batch.plan_batch(scs, W, H, sm)
+        rec, srec = packing.scene_records(scs, W, H, sm)
+        assert b"".join(bytes(s) for s in host[-1].scenes) == rec.tobytes()
+        if sm is not None:
+            assert b"".join(bytes(s) for s in host[0].scenes) == srec.tobytes()
+        else:
+            assert srec is None
