@@ -445,6 +445,14 @@ def main() -> None:
             traffic = measured[dom]["hbm_bytes"]
             valu_busy = measured[dom].get("valu_busy")
             prof_tile = measured[dom].get("tile", prof.get("tile"))
+            run_p = br.passes.get(dom.split(".")[0])
+            if isinstance(prof_tile, dict) and run_p is not None:
+                same = (list(prof_tile.get("tile", [])) == list(run_p.tile)
+                        and prof_tile.get("persistent") == run_p.persistent
+                        and prof_tile.get("tiles_per_group") == run_p.tiles_per_group
+                        and prof_tile.get("pixels_per_lane") == run_p.pixels_per_lane)
+                prof_tile = (f"profiled launch {prof_tile}: "
+                             f"{'the same as' if same else 'differs from'} this run's")
     if dom_fr is not None:
         best = dom_fr[dom_fr["bound"]]
         roofline = {"bound": dom_fr["bound"], "achieved": best["achieved"], "peak": best["peak"],
