@@ -235,3 +235,19 @@ def test_ant_like_frames_against_oracle():
     for s in range(3):
         # the walker itself (faces after the ground box's 12) is on screen
         assert ((pid[s] >= s * 3276 + 12) & (pid[s] < (s + 1) * 3276)).sum() > 200
+
+
+@pytest.mark.parametrize("tile,ppl", [((8, 8), 1), ((16, 16), 4)], ids=str)
+def test_darboux_head_512(tile, ppl):
+    """phong_darboux at scale: the reference examples' african head (2,492
+    triangles, tests/golden/african_head.obj) with a 512x512 diffuse map and a
+    tangent-space normal map, 512x512 pixels, against the oracle."""
+    from jaxrenderer_amd.pipeline import build_call
+    from jaxrenderer_amd.shaders import PhongTextureDarbouxShader
+    from tests import scenes
+
+    cam, faces, extra = scenes.darboux_head()
+    got, _ = _check_plan(lambda: [build_call(cam, PhongTextureDarbouxShader, faces, extra, 512,
+                                             512, init_buffers=True)],
+                         f"darboux-head/{tile}", tile=tile, pixels_per_lane=ppl)
+    assert int((as_np(got["main"]["prim_id"]) >= 0).sum()) > 40_000
