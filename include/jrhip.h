@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define JR_ABI_VERSION 1
+#define JR_ABI_VERSION 2
 
 /* Built-in shaders (renderer/shaders/ directory).  JR_SHADER_VISIBILITY runs only
  * the default primitive chooser (renderer/shader.py:155-251) and the default
@@ -209,6 +209,8 @@ typedef struct jr_grads {
   float* d_textures;    /* flat, like jr_inputs.textures */
   float* d_specular_maps;
   float* d_scene;       /* [n_scenes][JR_DSCENE_FLOATS]: see JR_DSCENE_* */
+  float* d_uvs;         /* [*,2] through uv_index (phong_darboux's tangent frame) */
+  float* d_normal_maps; /* flat, like jr_inputs.normal_maps (phong_darboux) */
 } jr_grads;
 
 /* Layout of one row of jr_grads.d_scene. */
@@ -299,8 +301,11 @@ int32_t jr_stage_times(float* ms, int32_t max_calls);
  * accumulates (+=) gradients into `grads`.  Replaces jax.grad through
  * renderer/pipeline.py:163-328 (edge functions, depth, interpolation), the
  * shaders' vertex and fragment stages and jnp.linalg.inv
- * (renderer/pipeline.py:105).  Supported shaders: DEPTH, GOURAUD,
- * GOURAUD_TEXTURE, PHONG, PHONG_REFLECTION, PHONG_REFLECTION_SHADOW.
+ * (renderer/pipeline.py:105).  Supported shaders: all seven built-ins
+ * (DEPTH, GOURAUD, GOURAUD_TEXTURE, PHONG, PHONG_DARBOUX, PHONG_REFLECTION,
+ * PHONG_REFLECTION_SHADOW); PHONG_DARBOUX also differentiates its tangent
+ * frame (renderer/shaders/phong_darboux.py:212-293: the face's NDC corners,
+ * their uvs, jnp.linalg.inv of the frame matrix) and its normal-map texel.
  * frame->width/height/n_scenes/shader/tile_w/tile_h must match the forward
  * call; the workspace is sized by jr_backward_workspace_size.  The workspace
  * carries state from call to call (its per-triangle accumulators are left
@@ -315,6 +320,20 @@ int32_t jr_backward(const jr_frame* frame, const jr_inputs* inputs,
                     const float* d_zbuffer, const int32_t* prim_id,
                     jr_grads* grads, void* workspace, int64_t workspace_bytes,
                     jr_stream_t stream);
+
+/* jr_backward of the image loss (scale / 2) sum((canvas - target)^2): the
+ * canvas cotangent of every pixel is scale * (canvas - target) (float32,
+ * subtraction first), computed where jr_backward would read d_canvas, so a
+ * training step needs no separate loss-gradient kernels or d_canvas buffer.
+ * canvas is the forward's output, target [n_scenes][width][height][3].  No
+ * counterpart in the reference, where jax.grad differentiates the loss and
+ * the renderer together; scale = 2 gives the gradient of sum((canvas -
+ * target)^2). */
+int32_t jr_backward_l2(const jr_frame* frame, const jr_inputs* inputs,
+                       const jr_scene* scenes, const float* canvas, const float* target,
+                       float scale, const float* d_zbuffer, const int32_t* prim_id,
+                       jr_grads* grads, void* workspace, int64_t workspace_bytes,
+                       jr_stream_t stream);
 
 /* Object poses of a batch: the per-object world transform of merge_objects
  * (renderer/model.py:447-555), for every object of every scene in one call,

@@ -17,6 +17,9 @@ from typing import Optional
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjrhip.so")
 
+# include/jrhip.h's JR_ABI_VERSION this binding is written for
+ABI_VERSION = 2
+
 # shader ids (include/jrhip.h)
 SHADER_DEPTH = 0
 SHADER_GOURAUD = 1
@@ -178,6 +181,8 @@ class JrGrads(ctypes.Structure):
         ("d_textures", c_vp),
         ("d_specular_maps", c_vp),
         ("d_scene", c_vp),
+        ("d_uvs", c_vp),
+        ("d_normal_maps", c_vp),
     ]
 
 
@@ -250,6 +255,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         P(JrFrame), P(JrInputs), c_vp, c_vp, c_vp, c_vp, P(JrGrads), c_vp, c_i64, c_vp
     ]
     lib.jr_backward.restype = c_i32
+    lib.jr_backward_l2.argtypes = [
+        P(JrFrame), P(JrInputs), c_vp, c_vp, c_vp, c_f, c_vp, c_vp, P(JrGrads), c_vp, c_i64, c_vp
+    ]
+    lib.jr_backward_l2.restype = c_i32
     lib.jr_backward_workspace_size.argtypes = [P(JrFrame), c_i32]
     lib.jr_backward_workspace_size.restype = c_i64
     lib.jr_backward_last_error.restype = ctypes.c_char_p
@@ -285,8 +294,8 @@ def lib() -> ctypes.CDLL:
         handle = ctypes.CDLL(LIB_PATH)
         _declare(handle)
         check_layout(handle)
-        if handle.jr_version() != 1:
-            raise ImportError(f"{LIB_PATH}: ABI version {handle.jr_version()} != 1")
+        if handle.jr_version() != ABI_VERSION:
+            raise ImportError(f"{LIB_PATH}: ABI version {handle.jr_version()} != {ABI_VERSION}")
         _lib = handle
     return _lib
 
@@ -294,6 +303,7 @@ def lib() -> ctypes.CDLL:
 def check(rc: int, what: str) -> None:
     if rc != 0:
         getter = {"jr_backward": lib().jr_backward_last_error,
+                  "jr_backward_l2": lib().jr_backward_last_error,
                   "jr_pack": lib().jr_pack_last_error}.get(what, lib().jr_last_error)
         msg = getter().decode(errors="replace")
         raise HipError(f"{what} failed with code {rc}: {msg}")

@@ -336,7 +336,7 @@ def workspace_size(*, width, height, n_scenes, n_tris, tile=DEFAULT_TILE, capaci
 
 
 _GRAD_FIELDS = ("d_positions", "d_colours", "d_normals", "d_textures", "d_specular_maps",
-                "d_scene")
+                "d_scene", "d_uvs", "d_normal_maps")
 
 
 def backward(
@@ -356,10 +356,15 @@ def backward(
     shadow_shape: Tuple[int, int] = (0, 0),
     stream: Optional[torch.cuda.Stream] = None,
     workspace: Optional[torch.Tensor] = None,
+    l2_canvas: Optional[torch.Tensor] = None,
+    l2_target: Optional[torch.Tensor] = None,
+    l2_scale: float = 2.0,
 ) -> None:
     """One jr_backward call: accumulates into the float32 tensors of `grads`
     (keys d_positions, d_colours, d_normals, d_textures, d_specular_maps,
-    d_scene)."""
+    d_scene, d_uvs, d_normal_maps).  With l2_target (and the forward's canvas
+    l2_canvas) in place of d_canvas it is one jr_backward_l2 call: the canvas
+    cotangent l2_scale * (canvas - target) is formed inside the kernel."""
     L = _hip.lib()
     dev = prim_id.device
     stream = stream or torch.cuda.current_stream(dev)
@@ -377,6 +382,24 @@ def backward(
     ws = workspace if workspace is not None else workspace_for(stream, need, dev, "backward")
     if ws.numel() < need:
         raise ValueError(f"backward workspace holds {ws.numel()} bytes, {need} needed")
+    if l2_target is not None:
+        if d_canvas is not None or l2_canvas is None:
+            raise ValueError("l2_target needs l2_canvas and no d_canvas")
+        shape = (n_scenes, width, height, 3)
+        for nm, t in (("l2_canvas", l2_canvas), ("l2_target", l2_target)):
+            if tuple(t.shape) != shape:
+                raise ValueError(f"{nm} must have shape {shape}, got {tuple(t.shape)}")
+        rc = L.jr_backward_l2(
+            ctypes.byref(fr), ctypes.byref(jin), ctypes.c_void_p(scenes.data_ptr()),
+            ctypes.c_void_p(_ptr(l2_canvas, "l2_canvas", torch.float32, dev)),
+            ctypes.c_void_p(_ptr(l2_target, "l2_target", torch.float32, dev)),
+            ctypes.c_float(l2_scale),
+            ctypes.c_void_p(_ptr(d_zbuffer, "d_zbuffer", torch.float32, dev)),
+            ctypes.c_void_p(_ptr(prim_id, "prim_id", torch.int32, dev)), ctypes.byref(jg),
+            ctypes.c_void_p(ws.data_ptr()), ctypes.c_int64(ws.numel()),
+            ctypes.c_void_p(stream.cuda_stream))
+        _hip.check(rc, "jr_backward_l2")
+        return
     rc = L.jr_backward(
         ctypes.byref(fr), ctypes.byref(jin), ctypes.c_void_p(scenes.data_ptr()),
         ctypes.c_void_p(_ptr(d_canvas, "d_canvas", torch.float32, dev)),

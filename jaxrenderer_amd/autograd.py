@@ -6,7 +6,8 @@ The reference differentiates its pipeline with jax.grad
 forward, keeps only the per-pixel primitive ids, and its backward calls the
 hand-written jr_backward kernels, which recompute every fragment from those
 ids.  Gradients reach vertex positions, normals, per-vertex colours, texture
-and specular maps, and -- through one per-scene parameter vector laid out like
+and specular maps (and for phong_darboux, whose tangent frame is built from
+each face's corners and uvs, the uvs and the normal map), and -- through one per-scene parameter vector laid out like
 include/jrhip.h's JR_DSCENE_* -- the light, the material coefficients, the
 shadow strength and the camera matrices (world_to_clip, viewport,
 world_to_eye_norm).  Visibility is piecewise constant, as under jax.grad.
@@ -60,23 +61,34 @@ def scenes_with_rows(scenes: Sequence[_hip.JrScene], rows: np.ndarray) -> list:
     return out
 
 
-_TENSOR_INPUTS = ("positions", "normals", "colours", "textures", "specular_maps")
+_TENSOR_INPUTS = ("positions", "normals", "colours", "textures", "specular_maps", "uvs",
+                  "normal_maps")
+_GRAD_NAMES = ("d_positions", "d_normals", "d_colours", "d_textures", "d_specular_maps",
+               "d_uvs", "d_normal_maps")
 
 
 class RenderFunction(torch.autograd.Function):
-    """(params [B, 69], positions, normals, colours, textures, specular_maps)
-    -> (zbuffer [B, W, H], canvas [B, W, H, 3])."""
+    """(params [B, 69], positions, normals, colours, textures, specular_maps,
+    uvs, normal_maps) -> (zbuffer [B, W, H], canvas [B, W, H, 3]).  uvs and
+    normal_maps receive gradients under phong_darboux only (elsewhere uvs
+    reach the image through floor alone)."""
 
     @staticmethod
     def forward(ctx, call: RasterCall, params, positions, normals, colours, textures,
-                specular_maps):
+                specular_maps, uvs=None, normal_maps=None):
         dev = params.device
         rows = params.detach().float().cpu().numpy()
         scenes = _raster.scenes_to_device(scenes_with_rows(call.scenes, rows), dev)
         inputs = {k: _plan.to_device(v, dev, k) for k, v in call.inputs.items()
                   if k not in _TENSOR_INPUTS}
-        for name, t in zip(_TENSOR_INPUTS, (positions, normals, colours, textures, specular_maps)):
-            inputs[name] = None if t is None else t.detach().float().contiguous()
+        tensors = (positions, normals, colours, textures, specular_maps, uvs, normal_maps)
+        for name, t in zip(_TENSOR_INPUTS, tensors):
+            if t is not None:
+                inputs[name] = t.detach().float().contiguous()
+            elif name in ("uvs", "normal_maps") and name in call.inputs:
+                inputs[name] = _plan.to_device(call.inputs[name], dev, name)
+            else:
+                inputs[name] = None
         B, W, H = call.n_scenes, call.width, call.height
         zbuffer = torch.empty((B, W, H), dtype=torch.float32, device=dev)
         canvas = (torch.empty((B, W, H, 3), dtype=torch.float32, device=dev)
@@ -92,8 +104,7 @@ class RenderFunction(torch.autograd.Function):
         ctx.scenes = scenes
         ctx.inputs = inputs
         ctx.prim_id = prim_id
-        ctx.shapes = [None if t is None else tuple(t.shape)
-                      for t in (positions, normals, colours, textures, specular_maps)]
+        ctx.shapes = [None if t is None else tuple(t.shape) for t in tensors]
         ctx.needs = ctx.needs_input_grad
         if canvas is None:
             canvas = torch.zeros((B, W, H, 3), dtype=torch.float32, device=dev)
@@ -104,8 +115,7 @@ class RenderFunction(torch.autograd.Function):
         call = ctx.call
         dev = ctx.prim_id.device
         grads: Dict[str, Optional[torch.Tensor]] = {}
-        names = ("d_positions", "d_normals", "d_colours", "d_textures", "d_specular_maps")
-        for i, (name, shape) in enumerate(zip(names, ctx.shapes)):
+        for i, (name, shape) in enumerate(zip(_GRAD_NAMES, ctx.shapes)):
             need = ctx.needs[2 + i] and shape is not None
             grads[name] = torch.zeros(shape, dtype=torch.float32, device=dev) if need else None
         grads["d_scene"] = (torch.zeros((call.n_scenes, _hip.DSCENE_FLOATS), dtype=torch.float32,
@@ -118,8 +128,15 @@ class RenderFunction(torch.autograd.Function):
                 n_scenes=call.n_scenes, inputs=ctx.inputs, n_slots=call.n_slots,
                 n_tris=call.n_tris, prim_id=ctx.prim_id, d_canvas=dc, d_zbuffer=dz,
                 grads=grads, shadow_shape=call.shadow_shape)
-        return (None, grads["d_scene"], grads["d_positions"], grads["d_normals"],
-                grads["d_colours"], grads["d_textures"], grads["d_specular_maps"])
+        return (None, grads["d_scene"]) + tuple(grads[n] for n in _GRAD_NAMES)
+
+
+def darboux_leaves(shader, leaf) -> tuple:
+    """(uvs, normal_maps) tensors for phong_darboux, whose image depends on
+    them continuously; (None, None) for the other shaders."""
+    if shader._jr_shader_id != _hip.SHADER_PHONG_DARBOUX:
+        return (None, None)
+    return (leaf("uv"), leaf("normal_map"))
 
 
 def _as_tensor(x, dev) -> torch.Tensor:
@@ -208,8 +225,6 @@ def render_views_differentiable(cameras: Sequence, shader, face_indices, extra, 
     tensor leaves of `extra`."""
     if not is_builtin(shader):
         raise NotImplementedError("render_views_differentiable supports the built-in shaders")
-    if shader._jr_shader_id == _hip.SHADER_PHONG_DARBOUX:
-        raise NotImplementedError("phong_darboux has no backward pass")
     dev = _raster.require_device()
 
     def host(x):
@@ -231,7 +246,7 @@ def render_views_differentiable(cameras: Sequence, shader, face_indices, extra, 
         return None if v is None else _as_tensor(v, dev).float()
 
     return RenderFunction.apply(call, params, leaf("position"), leaf("normal"), leaf("colour"),
-                                leaf("texture"), leaf("specular_map"))
+                                leaf("texture"), leaf("specular_map"), *darboux_leaves(shader, leaf))
 
 
 def render_differentiable(camera, shader, face_indices, extra, width: int, height: int, *,
@@ -242,8 +257,6 @@ def render_differentiable(camera, shader, face_indices, extra, width: int, heigh
     canvas [W, H, 3])."""
     if not is_builtin(shader):
         raise NotImplementedError("render_differentiable supports the built-in shaders")
-    if shader._jr_shader_id == _hip.SHADER_PHONG_DARBOUX:
-        raise NotImplementedError("phong_darboux has no backward pass")
     dev = _raster.require_device()
 
     def host(x):
@@ -264,7 +277,8 @@ def render_differentiable(camera, shader, face_indices, extra, width: int, heigh
         return None if v is None else _as_tensor(v, dev).float()
 
     zb, cv = RenderFunction.apply(call, params, leaf("position"), leaf("normal"), leaf("colour"),
-                                  leaf("texture"), leaf("specular_map"))
+                                  leaf("texture"), leaf("specular_map"),
+                                  *darboux_leaves(shader, leaf))
     return zb[0], cv[0]
 
 

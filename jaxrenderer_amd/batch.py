@@ -435,12 +435,27 @@ class BatchRenderer:
             shadow_shape=c.shadow_shape, workspace=self.bwd_workspace)
         return self.grads
 
+    def backward_l2(self, target: torch.Tensor) -> Dict[str, torch.Tensor]:
+        """Gradients of the L2 image loss sum((canvas - target)^2) of the last
+        render(), written into self.grads: one jr_backward_l2 call, which
+        forms the canvas cotangent 2 (canvas - target) per pixel itself."""
+        main = self.passes["main"]
+        c = main.call
+        for g in self.grads.values():
+            g.zero_()
+        _raster.backward(
+            shader=c.shader, width=c.width, height=c.height, scenes=main.scenes,
+            n_scenes=c.n_scenes, inputs=main.inputs, n_slots=c.n_slots, n_tris=c.n_tris,
+            prim_id=main.prim_id, d_canvas=None, d_zbuffer=None, grads=self.grads,
+            shadow_shape=c.shadow_shape, workspace=self.bwd_workspace,
+            l2_canvas=main.canvas, l2_target=target, l2_scale=2.0)
+        return self.grads
+
     def train_step(self, target: torch.Tensor) -> Dict[str, torch.Tensor]:
         """Forward, the gradient of the L2 image loss sum((canvas - target)^2),
         and backward."""
-        out = self.render_overlapped()
-        self._d_canvas = 2.0 * (out["canvas"] - target)
-        return self.backward(self._d_canvas)
+        self.render_overlapped()
+        return self.backward_l2(target)
 
     def capture_train(self, target: torch.Tensor) -> torch.cuda.CUDAGraph:
         """Captures train_step(target) in one HIP graph."""

@@ -41,10 +41,28 @@
 
 namespace jr {
 
-constexpr int kAcc = 21;   // dMinv[3][3] (row r, column k), dz_clip[3], dvar[3][3]
+// Per-triangle accumulators: dMinv[3][3] (row r, column k), dz_clip[3],
+// dvar[3][3]; phong_darboux adds the cotangents of its face's NDC corners
+// (3 x 3) and of their uvs (3 x 2).
+constexpr int kAcc = 21;
 constexpr int A_MINV = 0;
 constexpr int A_Z = 9;
 constexpr int A_VAR = 12;
+constexpr int A_TRI = 21;
+constexpr int A_TUV = 30;
+template <int SHADER>
+struct AccOf {
+  static constexpr int N = kAcc;
+};
+template <>
+struct AccOf<JR_SHADER_PHONG_DARBOUX> {
+  static constexpr int N = kAcc + 15;
+};
+
+// Accumulator floats per triangle of a shader (host side).
+inline int acc_floats(int shader) {
+  return shader == JR_SHADER_PHONG_DARBOUX ? AccOf<JR_SHADER_PHONG_DARBOUX>::N : kAcc;
+}
 
 // per-scene gradients reduced in k_bwd_pixels
 enum {
@@ -95,11 +113,14 @@ struct PixGrad {
   float db[3];
   float dvar[3][3];
   float sg[S_COUNT];
-  // the texel and specular-map entry this pixel's gradient lands on (-1:
-  // none); scattered after the per-pixel work, with the wavefront converged
-  int64_t tex_at, spec_at;
+  // the texel, specular-map and normal-map entry this pixel's gradient lands
+  // on (-1: none); scattered after the per-pixel work, with the wavefront
+  // converged
+  int64_t tex_at, spec_at, nmap_at;
   int32_t tex_n;
-  float tex_g[3], spec_g;
+  float tex_g[3], spec_g, nmap_g[3];
+  // phong_darboux: cotangents of the face's NDC corners and their uvs
+  float dtri[9], dtuv[6];
 };
 
 // A texel lookup transposes to an add at the same (already clamped) index.
@@ -126,16 +147,28 @@ struct ShadeBwd<JR_SHADER_DEPTH> {
                          const float*, const float*, PixGrad&, int) {}
 };
 
+// The shaders whose corner stage a tile stages once per triangle
+// (Varyings<SHADER>): run() computes it, run_v() takes it.
+template <int SHADER>
+struct CornerBwd {
+  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
+                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
+                         int scene) {
+    float v[Varyings<SHADER>::N];
+    Varyings<SHADER>::corners(in, sc, c, v);
+    ShadeBwd<SHADER>::run_v(in, sc, gr, v, b, dcol, g, scene);
+  }
+};
+
 // shaders/gouraud.py:90-124: col = sum_k b_k col_k
 template <>
-struct ShadeBwd<JR_SHADER_GOURAUD> {
-  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads&,
-                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
-                         int) {
-    float v[3][3];
-    for (int k = 0; k < 3; ++k) gouraud_vertex(in, sc, c.slot[k], v[k]);
+struct ShadeBwd<JR_SHADER_GOURAUD> : CornerBwd<JR_SHADER_GOURAUD> {
+  // v: the three corners' lit colours
+  static JR_DEV void run_v(const jr_inputs&, const jr_scene&, const jr_grads&,
+                           const float* v, const float b[3], const float dcol[3], PixGrad& g,
+                           int) {
     for (int k = 0; k < 3; ++k) {
-      g.db[k] = dot3(dcol, v[k]);
+      g.db[k] = dot3(dcol, v + 3 * k);
       for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dcol[q];
     }
   }
@@ -143,36 +176,28 @@ struct ShadeBwd<JR_SHADER_GOURAUD> {
 
 // shaders/gouraud_texture.py:103-144: col = texture[uv] * sum_k b_k lc_k
 template <>
-struct ShadeBwd<JR_SHADER_GOURAUD_TEXTURE> {
-  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
-                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
-                         int) {
-    float lc[3][3], uvv[3][2], ln[3];
-    normalise3(sc.light_direction, ln);
-    for (int k = 0; k < 3; ++k) {
-      float n[3], nn[3];
-      load3(in.normals, attr_row(in.nrm_index, c.slot[k]), n);
-      normalise3(n, nn);
-      const float it = dot3(nn, ln);
-      for (int q = 0; q < 3; ++q) lc[k][q] = sc.light_colour[q] * it;
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
-    }
+struct ShadeBwd<JR_SHADER_GOURAUD_TEXTURE> : CornerBwd<JR_SHADER_GOURAUD_TEXTURE> {
+  // v: the corners' light colours lc_k (3 x 3), then their uvs (3 x 2)
+  static JR_DEV void run_v(const jr_inputs& in, const jr_scene& sc, const jr_grads&,
+                           const float* v, const float b[3], const float dcol[3], PixGrad& g,
+                           int) {
+    const float* uvv = v + 9;
     float light[3];
-    for (int q = 0; q < 3; ++q) light[q] = interp(b, lc[0][q], lc[1][q], lc[2][q]);
-    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    for (int q = 0; q < 3; ++q) light[q] = interp(b, v[q], v[3 + q], v[6 + q]);
+    const float uu = interp(b, uvv[0], uvv[2], uvv[4]);
+    const float vv = interp(b, uvv[1], uvv[3], uvv[5]);
     const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
-    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    const int32_t t = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
     float tex[3];
-    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, t, tex);
     float dt[3], dl[3];
     for (int q = 0; q < 3; ++q) {
       dt[q] = dcol[q] * light[q];
       dl[q] = dcol[q] * tex[q];
     }
-    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, t, dt);
     for (int k = 0; k < 3; ++k) {
-      g.db[k] = dot3(dl, lc[k]);
+      g.db[k] = dot3(dl, v + 3 * k);
       for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dl[q];
     }
   }
@@ -180,23 +205,21 @@ struct ShadeBwd<JR_SHADER_GOURAUD_TEXTURE> {
 
 // shaders/phong.py:106-164
 template <>
-struct ShadeBwd<JR_SHADER_PHONG> {
-  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
-                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
-                         int) {
-    float ne[3][3], uvv[3][2];
-    for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], ne[k]);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
-    }
+struct ShadeBwd<JR_SHADER_PHONG> : CornerBwd<JR_SHADER_PHONG> {
+  // v: the corners' eye-space normals (3 x 3), then their uvs (3 x 2)
+  static JR_DEV void run_v(const jr_inputs& in, const jr_scene& sc, const jr_grads&,
+                           const float* v, const float b[3], const float dcol[3], PixGrad& g,
+                           int) {
+    const float* ne = v;
+    const float* uvv = v + 9;
     float n[3];
-    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
-    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[q], ne[3 + q], ne[6 + q]);
+    const float uu = interp(b, uvv[0], uvv[2], uvv[4]);
+    const float vv = interp(b, uvv[1], uvv[3], uvv[5]);
     const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
-    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    const int32_t t = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
     float tex[3], nn[3], ln[3];
-    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, t, tex);
     normalise3(n, nn);
     normalise3(sc.light_direction, ln);
     const float d = dot3(nn, ln);
@@ -215,7 +238,7 @@ struct ShadeBwd<JR_SHADER_PHONG> {
       g.sg[S_LIGHT_COLOUR + q] += dlc[q] * d;
       dd += dlc[q] * sc.light_colour[q];
     }
-    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, t, dt);
     float dnn[3], dln[3], dn[3], dL[3];
     for (int q = 0; q < 3; ++q) {
       dnn[q] = dd * ln[q];
@@ -225,8 +248,174 @@ struct ShadeBwd<JR_SHADER_PHONG> {
     for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR + q] += dL[q];
     normalise_bwd(n, dnn, dn);
     for (int k = 0; k < 3; ++k) {
-      g.db[k] = dot3(dn, ne[k]);
+      g.db[k] = dot3(dn, ne + 3 * k);
       for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dn[q];
+    }
+  }
+};
+
+// shaders/phong_darboux.py:116-293: the forward of Shade<PHONG_DARBOUX>
+// with its intermediates, then its reverse.  Beyond phong's terms the
+// gradient reaches the tangent frame: B = [i/|i|, j/|j|, n] with
+// (i, j) = inv(A) @ [du, dv] and A = [t1 - t0; t2 - t0; n] built from the
+// face's NDC corners t_m = to_cartesian(world_to_clip @ [p_m, 1]) and their
+// uvs; d(inv(A)) transposes to dA = -inv(A)^T G inv(A)^T.  The corners'
+// cotangents go to the per-triangle accumulators (k_bwd_tris takes them to
+// positions, uvs and world_to_clip); the normal-map texel gets d(B @ nm).
+template <>
+struct ShadeBwd<JR_SHADER_PHONG_DARBOUX> {
+  static JR_DEV void run(const jr_inputs& in, const jr_scene& sc, const jr_grads& gr,
+                         const Corner& c, const float b[3], const float dcol[3], PixGrad& g,
+                         int) {
+    float ne[3][3], uvv[3][2];
+    for (int k = 0; k < 3; ++k) {
+      eye_normal(in, sc, c.slot[k], ne[k]);
+      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
+    }
+    const int32_t face = in.id_to_face[c.slot[0]];
+    float tri[3][3], tuv[3][2];
+    for (int m = 0; m < 3; ++m) {
+      const int32_t q = in.face_slots[3 * (int64_t)face + m];
+      float h[4];
+      position_clip(in, sc.world_to_clip, q, h);
+      const bool wz = (h[3] == 0.0f);
+      tri[m][0] = wz ? h[0] : h[0] / h[3];
+      tri[m][1] = wz ? h[1] : h[1] / h[3];
+      tri[m][2] = wz ? h[2] : h[2] / h[3];
+      load2(in.uvs, attr_row(in.uv_index, q), tuv[m]);
+    }
+    float n_i[3];
+    for (int q = 0; q < 3; ++q) n_i[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
+    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
+    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
+    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
+    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    float n[3];
+    normalise3(n_i, n);
+    float A[9], AI[9];
+    for (int q = 0; q < 3; ++q) {
+      A[q] = tri[1][q] - tri[0][q];
+      A[3 + q] = tri[2][q] - tri[0][q];
+      A[6 + q] = n[q];
+    }
+    inv3(A, AI);
+    const float du1 = tuv[1][0] - tuv[0][0], du2 = tuv[2][0] - tuv[0][0];
+    const float dv1 = tuv[1][1] - tuv[0][1], dv2 = tuv[2][1] - tuv[0][1];
+    float iv[3], jv[3];
+    for (int r = 0; r < 3; ++r) {
+      float s = AI[3 * r + 0] * du1;
+      s = s + AI[3 * r + 1] * du2;
+      s = s + AI[3 * r + 2] * 0.0f;
+      iv[r] = s;
+      float t = AI[3 * r + 0] * dv1;
+      t = t + AI[3 * r + 1] * dv2;
+      t = t + AI[3 * r + 2] * 0.0f;
+      jv[r] = t;
+    }
+    float in_[3], jn[3];
+    normalise3(iv, in_);
+    normalise3(jv, jn);
+    const int32_t un = index_clamp(u, sc.nmap_w);
+    const int32_t vn = index_clamp(v, sc.nmap_h);
+    float nm[3];
+    load3(in.normal_maps + sc.nmap_base, un * sc.nmap_h + vn, nm);
+    float bn[3];
+    for (int r = 0; r < 3; ++r) {
+      float s = in_[r] * nm[0];
+      s = s + jn[r] * nm[1];
+      s = s + n[r] * nm[2];
+      bn[r] = s;
+    }
+    float n2[3], ln[3];
+    normalise3(bn, n2);
+    normalise3(sc.light_direction, ln);
+    const float d = dot3(n2, ln);
+    float tex[3];
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
+    float lc[3];
+    bool pos = true;
+    for (int ch = 0; ch < 3; ++ch) {
+      lc[ch] = sc.light_colour[ch] * d;
+      pos = pos && (lc[ch] >= 0.0f);
+    }
+    if (!pos) return;  // the zero branch of the cond carries no gradient
+
+    // colour = tex * light_colour * dot(n2, ln)
+    float dt[3], dd = 0.0f;
+    for (int q = 0; q < 3; ++q) {
+      dt[q] = dcol[q] * lc[q];
+      const float dlc = dcol[q] * tex[q];
+      g.sg[S_LIGHT_COLOUR + q] += dlc * d;
+      dd += dlc * sc.light_colour[q];
+    }
+    add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
+    float dn2[3], dln[3], dL[3], dbn[3];
+    for (int q = 0; q < 3; ++q) {
+      dn2[q] = dd * ln[q];
+      dln[q] = dd * n2[q];
+    }
+    normalise_bwd(sc.light_direction, dln, dL);
+    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR + q] += dL[q];
+    normalise_bwd(bn, dn2, dbn);
+    // bn = B @ nm with B's columns in_, jn, n
+    float din[3], djn[3], dn[3];
+    for (int r = 0; r < 3; ++r) {
+      din[r] = dbn[r] * nm[0];
+      djn[r] = dbn[r] * nm[1];
+      dn[r] = dbn[r] * nm[2];
+    }
+    if (gr.d_normal_maps) {
+      g.nmap_at = sc.nmap_base + 3 * ((int64_t)un * sc.nmap_h + vn);
+      g.nmap_g[0] = dot3(dbn, in_);
+      g.nmap_g[1] = dot3(dbn, jn);
+      g.nmap_g[2] = dot3(dbn, n);
+    }
+    float div[3], djv[3];
+    normalise_bwd(iv, din, div);
+    normalise_bwd(jv, djn, djv);
+    // (i, j) = AI @ (du, dv): dAI = div du^T + djv dv^T, d(du) = AI^T div
+    float dAI[9];
+    float ddu1 = 0.0f, ddu2 = 0.0f, ddv1 = 0.0f, ddv2 = 0.0f;
+    for (int r = 0; r < 3; ++r) {
+      dAI[3 * r + 0] = div[r] * du1 + djv[r] * dv1;
+      dAI[3 * r + 1] = div[r] * du2 + djv[r] * dv2;
+      dAI[3 * r + 2] = div[r] * 0.0f + djv[r] * 0.0f;
+      ddu1 += div[r] * AI[3 * r + 0];
+      ddu2 += div[r] * AI[3 * r + 1];
+      ddv1 += djv[r] * AI[3 * r + 0];
+      ddv2 += djv[r] * AI[3 * r + 1];
+    }
+    g.dtuv[0] = -(ddu1 + ddu2);  // corner 0: u, v
+    g.dtuv[1] = -(ddv1 + ddv2);
+    g.dtuv[2] = ddu1;  // corner 1
+    g.dtuv[3] = ddv1;
+    g.dtuv[4] = ddu2;  // corner 2
+    g.dtuv[5] = ddv2;
+    // dA = -AI^T dAI AI^T
+    float T1[9], dA[9];
+    for (int p = 0; p < 3; ++p)
+      for (int k = 0; k < 3; ++k) {
+        float s = 0.0f;
+        for (int r = 0; r < 3; ++r) s += AI[3 * r + p] * dAI[3 * r + k];
+        T1[3 * p + k] = s;
+      }
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) {
+        float s = 0.0f;
+        for (int k = 0; k < 3; ++k) s += T1[3 * p + k] * AI[3 * q + k];
+        dA[3 * p + q] = -s;
+      }
+    for (int q = 0; q < 3; ++q) {
+      g.dtri[0 + q] = -(dA[q] + dA[3 + q]);
+      g.dtri[3 + q] = dA[q];
+      g.dtri[6 + q] = dA[3 + q];
+      dn[q] += dA[6 + q];
+    }
+    float dni[3];
+    normalise_bwd(n_i, dn, dni);
+    for (int k = 0; k < 3; ++k) {
+      g.db[k] = dot3(dni, ne[k]);
+      for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dni[q];
     }
   }
 };
@@ -372,18 +561,57 @@ JR_DEV float wave_sum(float v) {
 // entries shared by the tile's pixels, so a texel that several pixels read
 // costs one global atomic per channel and tile; indices beyond the table's
 // capacity (or beyond 2^31) are added directly.
-template <int CAP, int NC>
+// Sums a lane could not stage (its table full), kept in registers (at most
+// KMAX, later ones add at once) and added to global memory after the pixel
+// loop, so that no global atomic sits between the loop's loads (a
+// wavefront's memory operations complete in order, so a later load would
+// wait for the atomic's round trip).  A 16x16 tile whose 256 pixels read 256
+// distinct texels fills a 128-entry table after two of a lane's four pixels,
+// which leaves two per lane.
+template <int NC, int KMAX>
+struct Pending {
+  int64_t at[KMAX];
+  float v[KMAX][NC];
+  JR_DEV Pending() {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) at[k] = -1;
+  }
+  // false when every register entry is taken
+  JR_DEV bool push(int64_t a, int32_t n, const float* g) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (at[k] < 0) {
+        at[k] = a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[k][c] = c < n ? g[c] : 0.0f;
+        return true;
+      }
+    }
+    return false;
+  }
+  JR_DEV void flush(float* __restrict__ d) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+      if (at[k] >= 0)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (v[k][c] != 0.0f) atomicAdd(d + at[k] + c, v[k][c]);
+  }
+};
+
+template <int CAP, int NC, int KMAX>
 JR_DEV void stage_add(int32_t* tab, float (*val)[NC], float* __restrict__ d, int64_t at,
-                      int32_t n, const float* v) {
+                      int32_t n, const float* v, Pending<NC, KMAX>& pend) {
   if (!d || at < 0) return;
   const int32_t e = at < 0x7FFFFFFFLL ? table_insert(tab, CAP, (int32_t)at) : -1;
-  for (int c = 0; c < n; ++c) {
-    if (v[c] == 0.0f) continue;
-    if (e >= 0)
-      atomicAdd(&val[e][c], v[c]);
-    else
-      atomicAdd(d + at + c, v[c]);
+  if (e < 0) {
+    if (!pend.push(at, n, v))
+      for (int c = 0; c < n; ++c)
+        if (v[c] != 0.0f) atomicAdd(d + at + c, v[c]);
+    return;
   }
+  for (int c = 0; c < n; ++c)
+    if (v[c] != 0.0f) atomicAdd(&val[e][c], v[c]);
 }
 
 // Adds the staged sums to d (n channels per entry) and leaves the table free.
@@ -402,6 +630,11 @@ struct BwdArgs {
   const jr_scene* scenes;
   jr_grads gr;
   const float* d_canvas;
+  // fused L2 loss: with target set the canvas cotangent of a pixel is
+  // l2_scale * (canvas - target), read in place of d_canvas
+  const float* canvas;
+  const float* target;
+  float l2_scale;
   const float* d_zbuffer;
   const int32_t* prim_id;
   float* tacc;              // kAcc floats per triangle, zero between calls
@@ -415,7 +648,7 @@ struct BwdArgs {
 // Workspace header words.  The accumulators and flags are left zero by every
 // call (k_bwd_tris clears what it consumed), so a call only clears them in
 // full when the header does not describe this workspace yet.
-constexpr int kHdrMagicLo = 0, kHdrMagicHi = 1, kHdrTris = 2, kHdrFill = 3;
+constexpr int kHdrMagicLo = 0, kHdrMagicHi = 1, kHdrTris = 2, kHdrFill = 3, kHdrAcc = 4;
 constexpr uint32_t kMagicLo = 0x6a724277u, kMagicHi = 0x73706b31u;
 
 // One wavefront per 16x16 tile, lane l owning the 2x2 block of pixels
@@ -439,12 +672,18 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   constexpr int NV = Varyings<SHADER>::N;
   constexpr int NS = (12 + NV) | 1;  // record stride (odd: no LDS bank conflicts)
   __shared__ float trec[kBwdCap * NS];
-  __shared__ float acc[kBwdCap][kAcc + 1];
+  constexpr int ACC = AccOf<SHADER>::N;
+  constexpr bool DARBOUX = SHADER == JR_SHADER_PHONG_DARBOUX;
+  constexpr int kNmapCap = DARBOUX ? kTexCap : 1;  // distinct normal-map texels
+  constexpr int AS = ACC | 1;  // odd row stride: lanes of different rows hit different banks
+  __shared__ float acc[kBwdCap][AS];
   __shared__ int32_t tab[kBwdCap];
   __shared__ int32_t tex_tab[kTexCap];
   __shared__ float tex_val[kTexCap][3];
   __shared__ int32_t spec_tab[kSpecCap];
   __shared__ float spec_val[kSpecCap][1];
+  __shared__ int32_t nmap_tab[kNmapCap];
+  __shared__ float nmap_val[kNmapCap][3];
   const int lane = threadIdx.x;
   const int32_t gt = (int32_t)blockIdx.x;
   const int32_t s = udiv31(gt, a.T_m, a.T_sh);
@@ -471,6 +710,10 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
     spec_tab[e] = -1;
     spec_val[e][0] = 0.0f;
   }
+  for (int e = lane; e < kNmapCap; e += 64) {
+    nmap_tab[e] = -1;
+    nmap_val[e][0] = nmap_val[e][1] = nmap_val[e][2] = 0.0f;
+  }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -480,7 +723,7 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   }
   __syncthreads();
   if (lane < kBwdCap) {
-    for (int k = 0; k < kAcc; ++k) acc[lane][k] = 0.0f;
+    for (int k = 0; k < ACC; ++k) acc[lane][k] = 0.0f;
     const int32_t t = tab[lane];
     if (t >= 0) {
       TriSetup ts;
@@ -507,22 +750,27 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   // the lane's open runs: triangle (its table entry, or -1 beyond the table),
   // texel and specular-map entry, with their sums
   int32_t run_t = -1, run_slot = -1;
-  float run_v[kAcc];
-  for (int k = 0; k < kAcc; ++k) run_v[k] = 0.0f;
+  float run_v[ACC];
+  for (int k = 0; k < ACC; ++k) run_v[k] = 0.0f;
   int64_t run_tex = -1, run_spec = -1;
   int32_t run_tex_n = 0;
   float run_tex_g[3] = {0.0f, 0.0f, 0.0f}, run_spec_g = 0.0f;
+  int64_t run_nmap = -1;
+  float run_nmap_g[3] = {0.0f, 0.0f, 0.0f};
+  Pending<3, 2> pend_tex;
+  Pending<1, 1> pend_spec;
+  Pending<3, 2> pend_nmap;
   auto flush_tri = [&]() {
     if (run_t < 0) return;
     if (run_slot >= 0) {
-      for (int k = 0; k < kAcc; ++k)
+      for (int k = 0; k < ACC; ++k)
         if (run_v[k] != 0.0f) atomicAdd(&acc[run_slot][k], run_v[k]);
     } else {
       a.touched[run_t] = s + 1;  // k_bwd_tris only visits touched triangles
-      for (int k = 0; k < kAcc; ++k)
-        if (run_v[k] != 0.0f) atomicAdd(a.tacc + (int64_t)run_t * kAcc + k, run_v[k]);
+      for (int k = 0; k < ACC; ++k)
+        if (run_v[k] != 0.0f) atomicAdd(a.tacc + (int64_t)run_t * ACC + k, run_v[k]);
     }
-    for (int k = 0; k < kAcc; ++k) run_v[k] = 0.0f;
+    for (int k = 0; k < ACC; ++k) run_v[k] = 0.0f;
   };
 #pragma unroll 1
   for (int q = 0; q < 4; ++q) {
@@ -537,11 +785,16 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
     const int64_t pix = (int64_t)s * a.W * a.H + (int64_t)i * a.H + j;
     g.tex_at = -1;
     g.spec_at = -1;
+    g.nmap_at = -1;
+    if constexpr (DARBOUX) {
+      for (int k = 0; k < 9; ++k) g.dtri[k] = 0.0f;
+      for (int k = 0; k < 6; ++k) g.dtuv[k] = 0.0f;
+    }
     g.tex_n = 0;
     g.spec_g = 0.0f;
     for (int k = 0; k < 3; ++k) g.tex_g[k] = 0.0f;
-    float vals[kAcc];
-    for (int k = 0; k < kAcc; ++k) vals[k] = 0.0f;
+    float vals[ACC];
+    for (int k = 0; k < ACC; ++k) vals[k] = 0.0f;
     if (t >= 0) {
       float4 r0, r1, r2;
       float v[NV > 0 ? NV : 1];
@@ -569,12 +822,16 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
         g.db[k] = 0.0f;
         for (int c = 0; c < 3; ++c) g.dvar[k][c] = 0.0f;
       }
-      if (a.d_canvas) {
-        const float dcol[3] = {a.d_canvas[3 * pix], a.d_canvas[3 * pix + 1],
-                               a.d_canvas[3 * pix + 2]};
+      if (a.d_canvas || a.target) {
+        float dcol[3];
+        if (a.target) {
+          for (int c = 0; c < 3; ++c)
+            dcol[c] = a.l2_scale * (a.canvas[3 * pix + c] - a.target[3 * pix + c]);
+        } else {
+          for (int c = 0; c < 3; ++c) dcol[c] = a.d_canvas[3 * pix + c];
+        }
         if constexpr (NV > 0)
-          ReflectionBwd<SHADER == JR_SHADER_PHONG_REFLECTION_SHADOW>::run_v(
-              a.in, sc, a.gr, v, b, dcol, g, s);
+          ShadeBwd<SHADER>::run_v(a.in, sc, a.gr, v, b, dcol, g, s);
         else
           ShadeBwd<SHADER>::run(a.in, sc, a.gr, corner_slots(a.in, t), b, dcol, g, s);
       }
@@ -613,37 +870,62 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
       g.sg[S_VP11] += -dy * y / vp[5];
       for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) vals[A_VAR + 3 * k + c] = g.dvar[k][c];
-      for (int k = 0; k < kAcc; ++k) run_v[k] += vals[k];
+      if constexpr (DARBOUX) {
+        for (int k = 0; k < 9; ++k) vals[A_TRI + k] = g.dtri[k];
+        for (int k = 0; k < 6; ++k) vals[A_TUV + k] = g.dtuv[k];
+      }
+      for (int k = 0; k < ACC; ++k) run_v[k] += vals[k];
     }
     // texel and specular-map runs
     if (g.tex_at != run_tex) {
-      stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, run_tex, run_tex_n, run_tex_g);
+      stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, run_tex, run_tex_n, run_tex_g,
+                            pend_tex);
       run_tex = g.tex_at;
       run_tex_n = g.tex_n;
       run_tex_g[0] = run_tex_g[1] = run_tex_g[2] = 0.0f;
     }
     for (int c = 0; c < 3; ++c) run_tex_g[c] += g.tex_g[c];
     if (g.spec_at != run_spec) {
-      stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, run_spec, 1, &run_spec_g);
+      stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, run_spec, 1, &run_spec_g,
+                             pend_spec);
       run_spec = g.spec_at;
       run_spec_g = 0.0f;
     }
     run_spec_g += g.spec_g;
+    if constexpr (DARBOUX) {
+      if (g.nmap_at != run_nmap) {
+        stage_add<kNmapCap, 3>(nmap_tab, nmap_val, a.gr.d_normal_maps, run_nmap, 3, run_nmap_g,
+                               pend_nmap);
+        run_nmap = g.nmap_at;
+        run_nmap_g[0] = run_nmap_g[1] = run_nmap_g[2] = 0.0f;
+      }
+      for (int c = 0; c < 3; ++c) run_nmap_g[c] += g.nmap_g[c];
+    }
   }
   flush_tri();
-  stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, run_tex, run_tex_n, run_tex_g);
-  stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, run_spec, 1, &run_spec_g);
+  stage_add<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, run_tex, run_tex_n, run_tex_g,
+                        pend_tex);
+  stage_add<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, run_spec, 1, &run_spec_g,
+                         pend_spec);
+  if constexpr (DARBOUX)
+    stage_add<kNmapCap, 3>(nmap_tab, nmap_val, a.gr.d_normal_maps, run_nmap, 3, run_nmap_g,
+                           pend_nmap);
+  if (a.gr.d_textures) pend_tex.flush(a.gr.d_textures);
+  if (a.gr.d_specular_maps) pend_spec.flush(a.gr.d_specular_maps);
+  if constexpr (DARBOUX)
+    if (a.gr.d_normal_maps) pend_nmap.flush(a.gr.d_normal_maps);
   __syncthreads();
   // one texel's channels: tex_c is the scene's (one scene per tile)
   flush_adds<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, sc.tex_c > 1 ? 3 : 1);
   flush_adds<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, 1);
+  if constexpr (DARBOUX) flush_adds<kNmapCap, 3>(nmap_tab, nmap_val, a.gr.d_normal_maps, 3);
   if (lane < kBwdCap) {
     const int32_t t = tab[lane];
     if (t >= 0) {
       a.touched[t] = s + 1;
-      for (int k = 0; k < kAcc; ++k) {
+      for (int k = 0; k < ACC; ++k) {
         const float v = acc[lane][k];
-        if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * kAcc + k, v);
+        if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * ACC + k, v);
       }
     }
   }
@@ -733,11 +1015,12 @@ JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
   for (int q = 0; q < 3; ++q) dLd[q] = dLc[q] = 0.0f;
   if (__ballot(touched) == 0) return;  // nothing to add for this wavefront
   if (!touched) s = -1;
+  constexpr int ACC = AccOf<SHADER>::N;
   if (touched) {
-    float* A = a.tacc + (int64_t)t * kAcc;
+    float* A = a.tacc + (int64_t)t * ACC;
     bool any = false;
-    float v[kAcc];
-    for (int q = 0; q < kAcc; ++q) {
+    float v[ACC];
+    for (int q = 0; q < ACC; ++q) {
       v[q] = A[q];
       A[q] = 0.0f;
       any = any || (v[q] != 0.0f);
@@ -780,6 +1063,53 @@ JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
         }
         const float dv[3] = {v[A_VAR + 3 * k], v[A_VAR + 3 * k + 1], v[A_VAR + 3 * k + 2]};
         vertex_bwd<SHADER>(a, sc, slot, dv, dW, dE, dLd, dLc);
+      }
+      if constexpr (SHADER == JR_SHADER_PHONG_DARBOUX) {
+        // the face of slot 0 (id_to_face) whose NDC corners and uvs built the
+        // tangent frame: t_m = to_cartesian(h_m), h_m = world_to_clip @ [p_m, 1]
+        const int32_t face = a.in.id_to_face[face_slot(a.in, t, 0)];
+        for (int m = 0; m < 3; ++m) {
+          const int32_t q = a.in.face_slots[3 * (int64_t)face + m];
+          const float* dt = v + A_TRI + 3 * m;
+          if (dt[0] != 0.0f || dt[1] != 0.0f || dt[2] != 0.0f) {
+            const int32_t prow = attr_row(a.in.pos_index, q);
+            float p[3], h[4];
+            load3(a.in.positions, prow, p);
+            position_clip(a.in, sc.world_to_clip, q, h);
+            // where(w == 0, h[:3], h[:3] / w): x / w has d/dx = 1 / w and
+            // d/dw = -x / w^2
+            float dh[4];
+            if (h[3] == 0.0f) {
+              dh[0] = dt[0];
+              dh[1] = dt[1];
+              dh[2] = dt[2];
+              dh[3] = 0.0f;
+            } else {
+              float dw = 0.0f;
+              for (int c = 0; c < 3; ++c) {
+                dh[c] = dt[c] / h[3];
+                dw += -dt[c] * h[c] / (h[3] * h[3]);
+              }
+              dh[3] = dw;
+            }
+            const float ph[4] = {p[0], p[1], p[2], 1.0f};
+            for (int jj = 0; jj < 4; ++jj) {
+              float s2 = 0.0f;
+              for (int ii = 0; ii < 4; ++ii) {
+                s2 += sc.world_to_clip[4 * ii + jj] * dh[ii];
+                dW[4 * ii + jj] += dh[ii] * ph[jj];
+              }
+              if (jj < 3 && a.gr.d_positions)
+                atomicAdd(a.gr.d_positions + 3 * (int64_t)prow + jj, s2);
+            }
+          }
+          const float* du = v + A_TUV + 2 * m;
+          if (a.gr.d_uvs && (du[0] != 0.0f || du[1] != 0.0f)) {
+            const int32_t urow = attr_row(a.in.uv_index, q);
+            atomicAdd(a.gr.d_uvs + 2 * (int64_t)urow, du[0]);
+            atomicAdd(a.gr.d_uvs + 2 * (int64_t)urow + 1, du[1]);
+          }
+        }
       }
     }
   }
@@ -847,16 +1177,17 @@ void k_bwd_tris(BwdArgs a) {
 }
 
 // Reads the workspace header: if it does not describe this workspace (new,
-// resized, or used for another triangle count), asks k_bwd_fill to clear the
-// flags and accumulators.
-__global__ void k_bwd_prepare(uint32_t* __restrict__ hdr, int32_t n_tris) {
+// resized, or used for another triangle count or accumulator size), asks
+// k_bwd_fill to clear the flags and accumulators.
+__global__ void k_bwd_prepare(uint32_t* __restrict__ hdr, int32_t n_tris, int32_t acc) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const bool ok = hdr[kHdrMagicLo] == kMagicLo && hdr[kHdrMagicHi] == kMagicHi &&
-                  hdr[kHdrTris] == (uint32_t)n_tris;
+                  hdr[kHdrTris] == (uint32_t)n_tris && hdr[kHdrAcc] == (uint32_t)acc;
   hdr[kHdrFill] = ok ? 0u : 1u;
   hdr[kHdrMagicLo] = kMagicLo;
   hdr[kHdrMagicHi] = kMagicHi;
   hdr[kHdrTris] = (uint32_t)n_tris;
+  hdr[kHdrAcc] = (uint32_t)acc;
 }
 
 // Zeroes n 32-bit words after the header when k_bwd_prepare asked for it.
@@ -884,17 +1215,17 @@ void launch_bwd(const BwdArgs& a, hipStream_t st) {
 }
 
 // Workspace: header (256 B), touched flags (4 B per triangle), accumulators
-// (4 kAcc B per triangle).
+// (4 acc_floats(shader) B per triangle).
 struct BwdLayout {
   int64_t touched, tacc, total;
 };
 
-BwdLayout bwd_layout(int32_t n_tris) {
+BwdLayout bwd_layout(int32_t n_tris, int shader) {
   const int64_t n = std::max(n_tris, 1);
   BwdLayout L;
   L.touched = 256;
   L.tacc = L.touched + (4 * n + 255) / 256 * 256;
-  L.total = L.tacc + (4 * (int64_t)kAcc * n + 255) / 256 * 256;
+  L.total = L.tacc + (4 * (int64_t)acc_floats(shader) * n + 255) / 256 * 256;
   return L;
 }
 
@@ -906,34 +1237,44 @@ extern "C" {
 
 int64_t jr_backward_workspace_size(const jr_frame* frame, int32_t n_tris) {
   if (!frame || n_tris < 0) return -1;
-  return bwd_layout(n_tris).total;
+  return bwd_layout(n_tris, frame->shader).total;
 }
 
 const char* jr_backward_last_error(void) { return g_bwd_err.c_str(); }
 
-int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
-                    const float* d_canvas, const float* d_zbuffer, const int32_t* prim_id,
-                    jr_grads* grads, void* workspace, int64_t workspace_bytes,
-                    jr_stream_t stream_) {
+}  // extern "C"
+
+namespace jr {
+namespace {
+
+// jr_backward and jr_backward_l2: the canvas cotangent is d_canvas, or
+// l2_scale * (canvas - target) when target is set.
+int32_t backward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                      const float* d_canvas, const float* canvas, const float* target,
+                      float l2_scale, const float* d_zbuffer, const int32_t* prim_id,
+                      jr_grads* grads, void* workspace, int64_t workspace_bytes,
+                      jr_stream_t stream_) {
   if (!fr || !in || !scenes || !grads || !prim_id)
     return bfail(JR_E_INVALID, "frame, inputs, scenes, grads and prim_id are required");
   if (fr->width <= 0 || fr->height <= 0 || fr->n_scenes <= 0 || in->n_tris < 0)
     return bfail(JR_E_INVALID, "bad frame sizes");
   if (in->clip) return bfail(JR_E_UNSUPPORTED, "backward needs positions, not clip input");
   const int sh = fr->shader;
-  if (sh == JR_SHADER_PHONG_DARBOUX || sh == JR_SHADER_VISIBILITY || sh < 0 ||
-      sh >= JR_SHADER_COUNT)
+  if (sh == JR_SHADER_VISIBILITY || sh < 0 || sh >= JR_SHADER_COUNT)
     return bfail(JR_E_UNSUPPORTED, "shader has no backward pass");
-  if (!d_canvas && !d_zbuffer) return bfail(JR_E_INVALID, "no upstream gradient");
+  if (!d_canvas && !target && !d_zbuffer) return bfail(JR_E_INVALID, "no upstream gradient");
   if (!in->positions) return bfail(JR_E_INVALID, "positions are required");
   if (!in->faces && (int64_t)in->n_slots < 3 * (int64_t)in->n_tris)
     return bfail(JR_E_INVALID, "without faces, n_slots must be at least 3 n_tris");
   if (sh == JR_SHADER_GOURAUD && (!in->normals || !in->colours))
     return bfail(JR_E_INVALID, "gouraud needs normals and colours");
   if ((sh == JR_SHADER_GOURAUD_TEXTURE || sh == JR_SHADER_PHONG ||
-       sh == JR_SHADER_PHONG_REFLECTION || sh == JR_SHADER_PHONG_REFLECTION_SHADOW) &&
+       sh == JR_SHADER_PHONG_DARBOUX || sh == JR_SHADER_PHONG_REFLECTION ||
+       sh == JR_SHADER_PHONG_REFLECTION_SHADOW) &&
       (!in->normals || !in->uvs || !in->textures))
     return bfail(JR_E_INVALID, "textured shaders need normals, uvs and textures");
+  if (sh == JR_SHADER_PHONG_DARBOUX && (!in->id_to_face || !in->face_slots || !in->normal_maps))
+    return bfail(JR_E_INVALID, "phong_darboux needs id_to_face, face_slots and normal_maps");
   if ((sh == JR_SHADER_PHONG_REFLECTION || sh == JR_SHADER_PHONG_REFLECTION_SHADOW) &&
       (!in->tex_index || !in->texture_shapes || !in->specular_maps))
     return bfail(JR_E_INVALID, "phong_reflection needs tex_index, texture_shapes, specular_maps");
@@ -948,9 +1289,12 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
   a.scenes = scenes;
   a.gr = *grads;
   a.d_canvas = (sh == JR_SHADER_DEPTH) ? nullptr : d_canvas;
+  a.canvas = canvas;
+  a.target = (sh == JR_SHADER_DEPTH) ? nullptr : target;
+  a.l2_scale = l2_scale;
   a.d_zbuffer = d_zbuffer;
   a.prim_id = prim_id;
-  const BwdLayout WL = bwd_layout(in->n_tris);
+  const BwdLayout WL = bwd_layout(in->n_tris, sh);
   char* ws = static_cast<char*>(workspace);
   a.header = reinterpret_cast<uint32_t*>(ws);
   a.touched = reinterpret_cast<int32_t*>(ws + WL.touched);
@@ -967,7 +1311,8 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
   if (in->n_tris == 0) return JR_OK;
   // flags and accumulators: everything after the header
   const int64_t nwords = (WL.total - WL.touched) / 4;
-  hipLaunchKernelGGL(k_bwd_prepare, dim3(1), dim3(64), 0, st, a.header, in->n_tris);
+  hipLaunchKernelGGL(k_bwd_prepare, dim3(1), dim3(64), 0, st, a.header, in->n_tris,
+                     acc_floats(sh));
   hipLaunchKernelGGL(k_bwd_fill, dim3((unsigned)std::min<int64_t>((nwords + 255) / 256, 8192)),
                      dim3(256), 0, st, a.header,
                      reinterpret_cast<uint32_t*>(ws + WL.touched), nwords);
@@ -976,12 +1321,35 @@ int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* sce
     case JR_SHADER_GOURAUD: launch_bwd<JR_SHADER_GOURAUD>(a, st); break;
     case JR_SHADER_GOURAUD_TEXTURE: launch_bwd<JR_SHADER_GOURAUD_TEXTURE>(a, st); break;
     case JR_SHADER_PHONG: launch_bwd<JR_SHADER_PHONG>(a, st); break;
+    case JR_SHADER_PHONG_DARBOUX: launch_bwd<JR_SHADER_PHONG_DARBOUX>(a, st); break;
     case JR_SHADER_PHONG_REFLECTION: launch_bwd<JR_SHADER_PHONG_REFLECTION>(a, st); break;
     default: launch_bwd<JR_SHADER_PHONG_REFLECTION_SHADOW>(a, st); break;
   }
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return bfail(JR_E_HIP, hipGetErrorString(err));
   return JR_OK;
+}
+
+}  // namespace
+}  // namespace jr
+
+extern "C" {
+
+int32_t jr_backward(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                    const float* d_canvas, const float* d_zbuffer, const int32_t* prim_id,
+                    jr_grads* grads, void* workspace, int64_t workspace_bytes,
+                    jr_stream_t stream) {
+  return backward_body(fr, in, scenes, d_canvas, nullptr, nullptr, 0.0f, d_zbuffer, prim_id,
+                       grads, workspace, workspace_bytes, stream);
+}
+
+int32_t jr_backward_l2(const jr_frame* fr, const jr_inputs* in, const jr_scene* scenes,
+                       const float* canvas, const float* target, float scale,
+                       const float* d_zbuffer, const int32_t* prim_id, jr_grads* grads,
+                       void* workspace, int64_t workspace_bytes, jr_stream_t stream) {
+  if (!canvas || !target) return bfail(JR_E_INVALID, "canvas and target are required");
+  return backward_body(fr, in, scenes, nullptr, canvas, target, scale, d_zbuffer, prim_id,
+                       grads, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

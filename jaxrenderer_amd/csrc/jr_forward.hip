@@ -31,6 +31,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/jrhip.h"
@@ -353,11 +354,25 @@ constexpr int kSetupThreads = 256;
 struct SmallStage {
   float4 r[3][kSetupThreads];
   int4 box[kSetupThreads];
-  int32_t tri[kSetupThreads];
   int16_t off[kSetupThreads];   // exclusive prefix sum of the areas per wavefront
-  int16_t toff[kSetupThreads];  // exclusive prefix sum of the tile counts
-  uint32_t tmask[kSetupThreads];  // tiles that passed the test (bit q)
+  union {
+    struct {
+      int32_t tri[kSetupThreads];
+      int16_t toff[kSetupThreads];    // exclusive prefix sum of the tile counts
+      uint32_t tmask[kSetupThreads];  // tiles that passed the test (bit q)
+    } t;
+    // after the tile tests: the sample table of the workgroup's one scene,
+    // over toff and tmask (and over tri in the narrow form, which keeps no
+    // triangle ids in its keys)
+    struct {
+      int32_t tri_keep[kSetupThreads];
+      float x[(kSetupThreads * 6) / 4];
+    } smp_wide;
+    float smp_narrow[(kSetupThreads * 10) / 4];
+  } u;
 };
+constexpr int kSmpWide = (kSetupThreads * 6) / 4;
+constexpr int kSmpNarrow = (kSetupThreads * 10) / 4;
 
 // A triangle handed from the first half of k_setup to the second: its clip
 // corners (x, y, z, w) and (id, scene, det bits).
@@ -384,7 +399,13 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
   // workgroup's LDS would pass 20 KiB, an eighth of the CU's 160 KiB, and 7
   // workgroups would fit on a CU instead of 8
   __shared__ int8_t mark[kSetupThreads];
+  // lowest and highest scene among the workgroup's small triangles
+  __shared__ int32_t wg_scene[2];
   SmallStage& sm = stage.sm;
+  if (threadIdx.x == 0) {
+    wg_scene[0] = 0x7FFFFFFF;
+    wg_scene[1] = -1;
+  }
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
 
@@ -488,11 +509,13 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     if (area > 0 && !small) r = tiles_of(px, f);
     if (small) {
       n_small = (int32_t)area;
+      atomicMin(&wg_scene[0], s);
+      atomicMax(&wg_scene[1], s);
       sm.r[0][threadIdx.x] = r0;
       sm.r[1][threadIdx.x] = r1;
       sm.r[2][threadIdx.x] = r2;
       sm.box[threadIdx.x] = make_int4(px.tx0, px.ty0, px.ty1 - px.ty0 + 1, s);
-      sm.tri[threadIdx.x] = t;
+      sm.u.t.tri[threadIdx.x] = t;
     }
     const int64_t n_t =
         (r.tx0 > r.tx1) ? 0 : (int64_t)(r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
@@ -526,8 +549,8 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
   const int32_t total = __shfl(incl, 63, 64);
   const int32_t total_t = __shfl(incl_t, 63, 64);
   sm.off[threadIdx.x] = (int16_t)(incl - n_small);
-  sm.toff[threadIdx.x] = (int16_t)(incl_t - n_tile);
-  sm.tmask[threadIdx.x] = 0u;
+  sm.u.t.toff[threadIdx.x] = (int16_t)(incl_t - n_tile);
+  sm.u.t.tmask[threadIdx.x] = 0u;
   __syncthreads();
   // the tile test of every (triangle, tile) pair of the wavefront's counted
   // triangles: count the tiles that may hold a covered pixel and remember
@@ -537,45 +560,81 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     const int o = wbase + wave_owner(incl_t - n_tile, n_tile, base, mark + wbase, lane);
     if (item >= total_t) continue;
     const int4 box = sm.box[o];
-    const int32_t q = item - sm.toff[o];
+    const int32_t q = item - sm.u.t.toff[o];
     const int32_t so = box.w & 0x3FFFFFFF;
     const int32_t dq = div_small(q, box.z);
     const int32_t tx = box.x + dq, ty = box.y + (q - dq * box.z);
     if (tile_may_cover(sm.r[0][o], sm.r[1][o], sm.r[2][o], (box.w >> 30) != 0,
                        scenes[so].viewport, samples + (int64_t)so * (f.W + f.H), f, tx, ty)) {
       atomicAdd(counts + (int64_t)so * f.T + tx * f.tiles_y + ty, 1);
-      atomicOr(&sm.tmask[o], 1u << q);
+      atomicOr(&sm.u.t.tmask[o], 1u << q);
     }
   }
   __syncthreads();
-  if (n_tile > 0) masks[t] = sm.tmask[threadIdx.x];
-  // the tile raster's exact per-pixel arithmetic for every (triangle, pixel)
-  for (int32_t base = 0; base < total; base += 64) {
-    const int32_t item = base + lane;
-    const int o = wbase + wave_owner(incl - n_small, n_small, base, mark + wbase, lane);
-    if (item >= total) continue;
-    const int4 box = sm.box[o];
-    const int32_t q = item - sm.off[o];
-    // q / h for q < 64, h <= 64: a float estimate, corrected to the exact quotient
-    int32_t di = (int32_t)((float)q / (float)box.z);
-    if (di * box.z > q) --di;
-    if ((di + 1) * box.z <= q) ++di;
-    const int32_t i = box.x + di, j = box.y + (q - di * box.z);
-    const float4 q0 = sm.r[0][o], q1 = sm.r[1][o], q2 = sm.r[2][o];
-    const float* sx = samples + (int64_t)box.w * (f.W + f.H);
-    const float x = sx[i], y = sx[f.W + j];
-    const float e0 = edge(q0, x, y), e1 = edge(q1, x, y), e2 = edge(q2, x, y);
-    if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
-      const float* vp = scenes[box.w].viewport;
-      const uint32_t key = depth_key(win_depth(q0, q1, q2, e0, e1, e2, vp[10], vp[11]));
-      const int64_t pix = ((int64_t)box.w * f.W + i) * f.H + j;
-      if (f.narrow)
-        atomicMin(zkeys + pix, key);
-      else
-        atomicMin((unsigned long long*)(keys + pix),
-                  (unsigned long long)pack_key(key, sm.tri[o]));
-    }
+  if (n_tile > 0) masks[t] = sm.u.t.tmask[threadIdx.x];
+  // The sample table of the workgroup's small triangles' scene goes to LDS
+  // when they all share one scene and it fits: the loop below then reads no
+  // global memory, and its key atomics never hold up a load (a wavefront's
+  // memory operations complete in order).
+  const int32_t s_lo = wg_scene[0], s_hi = wg_scene[1];
+  const int32_t per = f.W + f.H;
+  const bool staged = s_lo == s_hi && per <= (f.narrow ? kSmpNarrow : kSmpWide);
+  float* smp = f.narrow ? sm.u.smp_narrow : sm.u.smp_wide.x;
+  float vz0 = 0.0f, vz1 = 0.0f;
+  if (staged) {
+    __syncthreads();  // every tmask read before the table overwrites it
+    const float* src = samples + (int64_t)s_lo * per;
+    for (int32_t k = threadIdx.x; k < per; k += kSetupThreads) smp[k] = src[k];
+    vz0 = scenes[s_lo].viewport[10];
+    vz1 = scenes[s_lo].viewport[11];
+    __syncthreads();
   }
+  // the tile raster's exact per-pixel arithmetic for every (triangle, pixel)
+  auto small_pairs = [&](auto from_lds) {
+    constexpr bool LDS = decltype(from_lds)::value;
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t item = base + lane;
+      const int o = wbase + wave_owner(incl - n_small, n_small, base, mark + wbase, lane);
+      if (item >= total) continue;
+      const int4 box = sm.box[o];
+      const int32_t q = item - sm.off[o];
+      // q / h for q < 64, h <= 64: a float estimate, corrected to the exact quotient
+      int32_t di = (int32_t)((float)q / (float)box.z);
+      if (di * box.z > q) --di;
+      if ((di + 1) * box.z <= q) ++di;
+      const int32_t i = box.x + di, j = box.y + (q - di * box.z);
+      const float4 q0 = sm.r[0][o], q1 = sm.r[1][o], q2 = sm.r[2][o];
+      float x, y;
+      if constexpr (LDS) {
+        x = smp[i];
+        y = smp[f.W + j];
+      } else {
+        const float* sx = samples + (int64_t)box.w * per;
+        x = sx[i];
+        y = sx[f.W + j];
+      }
+      const float e0 = edge(q0, x, y), e1 = edge(q1, x, y), e2 = edge(q2, x, y);
+      if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
+        float v0 = vz0, v1 = vz1;
+        if constexpr (!LDS) {
+          const float* vp = scenes[box.w].viewport;
+          v0 = vp[10];
+          v1 = vp[11];
+        }
+        const uint32_t key = depth_key(win_depth(q0, q1, q2, e0, e1, e2, v0, v1));
+        const int64_t pix = ((int64_t)box.w * f.W + i) * f.H + j;
+        if (f.narrow)
+          atomicMin(zkeys + pix, key);
+        else
+          atomicMin((unsigned long long*)(keys + pix),
+                    (unsigned long long)pack_key(key, sm.u.t.tri[o]));
+      }
+    }
+  };
+  if (staged)
+    small_pairs(std::true_type{});
+  else
+    small_pairs(std::false_type{});
 }
 
 // Counts (BIN = false) or bins (BIN = true) the wide triangles: one wavefront
@@ -843,18 +902,41 @@ JR_DEV float shadow_depth_on_demand(const ShadowLookup& L, int32_t s, float x, f
   const int64_t start = L.offsets[gt];
   const bool overflow = start < 0 || cnt < 0 || start + cnt > f.capacity;
   const int32_t n = overflow ? b1 - b0 : cnt;
-  for (int32_t m = 0; m < n; ++m) {
-    const int32_t t = overflow ? b0 + m : L.bins[start + m];
-    if (t < b0 || t >= b1) continue;
-    if (overflow && !lists_tile(L.rects, L.masks, t, tx, ty)) continue;
-    const float4 r0 = L.rec[4 * (int64_t)t], r1 = L.rec[4 * (int64_t)t + 1],
-                 r2 = L.rec[4 * (int64_t)t + 2];
-    const float e0 = edge(r0, px, py), e1 = edge(r1, px, py), e2 = edge(r2, px, py);
-    if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
-      const uint32_t key = depth_key(win_depth(r0, r1, r2, e0, e1, e2, vp22, vp23));
-      if (key < best_key || (key == best_key && t < best_id)) {
-        best_key = key;
-        best_id = t;
+  // two list entries per step: their ids, then their records, are loaded
+  // together (the (key, id) minimum does not depend on the visiting order)
+  for (int32_t m0 = 0; m0 < n; m0 += 2) {
+    int32_t tt[2];
+    float4 ra[2], rb[2], rc[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int32_t m = m0 + u;
+      int32_t t = -1;
+      if (m < n) t = overflow ? b0 + m : L.bins[start + m];
+      if (t < b0 || t >= b1) t = -1;
+      tt[u] = t;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      ra[u] = rb[u] = rc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (tt[u] >= 0) {
+        const int64_t tq = 4 * (int64_t)tt[u];
+        ra[u] = L.rec[tq];
+        rb[u] = L.rec[tq + 1];
+        rc[u] = L.rec[tq + 2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int32_t t = tt[u];
+      if (t < 0) continue;
+      if (overflow && !lists_tile(L.rects, L.masks, t, tx, ty)) continue;
+      const float e0 = edge(ra[u], px, py), e1 = edge(rb[u], px, py), e2 = edge(rc[u], px, py);
+      if (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {
+        const uint32_t key = depth_key(win_depth(ra[u], rb[u], rc[u], e0, e1, e2, vp22, vp23));
+        if (key < best_key || (key == best_key && t < best_id)) {
+          best_key = key;
+          best_id = t;
+        }
       }
     }
   }
@@ -1083,9 +1165,11 @@ JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t 
   using V = Varyings<SHADER>;
   constexpr int NS = V::N | 1;  // odd stride: no LDS bank conflicts
   constexpr int CAP0 = (48 * NT) / (4 * NS);
-  constexpr int CAP = CAP0 < NT ? CAP0 : NT;
+  constexpr int CAP = CAP0 < (NT - 1) / 2 ? CAP0 : (NT - 1) / 2;
   float* vary = reinterpret_cast<float*>(lds_rec);
   int32_t* tab = lds_id;
+  int32_t* dense = lds_id + CAP;        // the occupied entries of tab, packed
+  int32_t* n_dense = lds_id + NT - 1;   // their count
   PixelHit h[PPL];
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
@@ -1098,19 +1182,21 @@ JR_DEV void finish_tile_shared(const RasterArgs& a, const jr_scene& sc, int32_t 
   }
   tile_sync<WAVE>();  // the visibility loop's reads of the records are done
   for (int e = tid; e < CAP; e += NT) tab[e] = -1;
+  if (tid == 0) *n_dense = 0;
   tile_sync<WAVE>();
   int32_t slot[PPL];
 #pragma unroll
   for (int q = 0; q < PPL; ++q) slot[q] = h[q].cover ? table_insert(tab, CAP, h[q].w) : -1;
   tile_sync<WAVE>();
-  for (int e = tid; e < CAP; e += NT) {
-    const int32_t w = tab[e];
-    if (w >= 0) {
-      float v[V::N];
-      V::corners(a.in, sc, corner_slots(a.in, w), v);
-#pragma unroll
-      for (int k = 0; k < V::N; ++k) vary[e * NS + k] = v[k];
-    }
+  for (int e = tid; e < CAP; e += NT)
+    if (tab[e] >= 0) dense[atomicAdd(n_dense, 1)] = e;
+  tile_sync<WAVE>();
+  // one lane per corner of each distinct winning triangle
+  const int32_t nd = *n_dense;
+  for (int32_t idx = tid; idx < 3 * nd; idx += NT) {
+    const int32_t d = idx / 3, k = idx - 3 * d;
+    const int32_t e = dense[d];
+    V::corner(a.in, sc, corner_slots(a.in, tab[e]), k, vary + e * NS);
   }
   tile_sync<WAVE>();
   float l[3];
@@ -1679,7 +1765,7 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   int32_t* wave_scene = reinterpret_cast<int32_t*>(ws + L.wave_scene);
   int32_t* wide_count = misc + 2;
   uint32_t* wbits = reinterpret_cast<uint32_t*>(ws + L.wbits);
-  const unsigned wide_grid = 2048;  // 8192 wavefronts stride over the wide list
+  const unsigned wide_grid = 8192;  // 32768 wavefronts stride over the wide list
 
   // a binning-only call is timed too (its raster stage then takes no time)
   StageTimer timer(st, timed && fr->part != JR_PART_RASTER);

@@ -172,103 +172,8 @@ struct Shade<JR_SHADER_VISIBILITY> {
   }
 };
 
-// shaders/gouraud.py:90-124
-template <>
-struct Shade<JR_SHADER_GOURAUD> {
-  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc,
-                         const Corner& c, const float b[3], bool front,
-                         float col[3], int) {
-    float v0[3], v1[3], v2[3];
-    gouraud_vertex(in, sc, c.slot[0], v0);
-    gouraud_vertex(in, sc, c.slot[1], v1);
-    gouraud_vertex(in, sc, c.slot[2], v2);
-    bool pos = true;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      col[ch] = interp(b, v0[ch], v1[ch], v2[ch]);
-      pos = pos && (col[ch] >= 0.0f);
-    }
-    return front && pos;
-  }
-};
 
-// shaders/gouraud_texture.py:103-144
-template <>
-struct Shade<JR_SHADER_GOURAUD_TEXTURE> {
-  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc,
-                         const Corner& c, const float b[3], bool front,
-                         float col[3], int) {
-    float lc[3][3], uvv[3][2];
-    float nn_l[3];
-    normalise3(sc.light_direction, nn_l);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float n[3], nn[3];
-      load3(in.normals, attr_row(in.nrm_index, c.slot[k]), n);
-      normalise3(n, nn);
-      const float intensity = dot3(nn, nn_l);
-      lc[k][0] = sc.light_colour[0] * intensity;
-      lc[k][1] = sc.light_colour[1] * intensity;
-      lc[k][2] = sc.light_colour[2] * intensity;
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
-    }
-    float light[3];
-    bool pos = true;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      light[ch] = interp(b, lc[0][ch], lc[1][ch], lc[2][ch]);
-      pos = pos && (light[ch] >= 0.0f);
-    }
-    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
-    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
-    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
-    float tex[3];
-    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
-    col[0] = tex[0] * light[0];
-    col[1] = tex[1] * light[1];
-    col[2] = tex[2] * light[2];
-    return front && pos;
-  }
-};
 
-// shaders/phong.py:106-164
-template <>
-struct Shade<JR_SHADER_PHONG> {
-  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc,
-                         const Corner& c, const float b[3], bool front,
-                         float col[3], int) {
-    float ne[3][3], uvv[3][2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], ne[k]);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), uvv[k]);
-    }
-    float n[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[0][q], ne[1][q], ne[2][q]);
-    const float uu = interp(b, uvv[0][0], uvv[1][0], uvv[2][0]);
-    const float vv = interp(b, uvv[0][1], uvv[1][1], uvv[2][1]);
-    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
-    const int32_t v = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
-    float tex[3];
-    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, v, tex);
-    float nn[3], ln[3];
-    normalise3(n, nn);
-    normalise3(sc.light_direction, ln);
-    const float d = dot3(nn, ln);
-    float lc[3];
-    bool pos = true;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      lc[ch] = sc.light_colour[ch] * d;
-      pos = pos && (lc[ch] >= 0.0f);
-    }
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) col[ch] = pos ? tex[ch] * lc[ch] : 0.0f;
-    return front;
-  }
-};
 
 // shaders/phong_darboux.py:116-293
 template <>
@@ -373,6 +278,144 @@ struct Varyings {
   static constexpr int N = 0;
 };
 
+// shaders/gouraud.py:58-124: per corner the lit vertex colour.
+template <>
+struct Varyings<JR_SHADER_GOURAUD> {
+  static constexpr int N = 9;
+  // corner k's entries of v (another lane may write the other corners')
+  static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
+                            float* v) {
+    gouraud_vertex(in, sc, c.slot[k], v + 3 * k);
+  }
+  static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                             float v[N]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) corner(in, sc, c, k, v);
+  }
+  static JR_DEV void scene_terms(const jr_scene&, float l[3]) { l[0] = l[1] = l[2] = 0.0f; }
+  static JR_DEV bool pixel(const jr_inputs&, const jr_scene&, const float v[N], const float*,
+                           const float b[3], bool front, float col[3], int) {
+    bool pos = true;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      col[ch] = interp(b, v[ch], v[3 + ch], v[6 + ch]);
+      pos = pos && (col[ch] >= 0.0f);
+    }
+    return front && pos;
+  }
+};
+
+// shaders/gouraud_texture.py:68-144: per corner the light colour times the
+// lambertian intensity, and the uv.
+template <>
+struct Varyings<JR_SHADER_GOURAUD_TEXTURE> {
+  static constexpr int N = 15;
+  static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
+                            float* v) {
+    float nn_l[3], n[3], nn[3];
+    normalise3(sc.light_direction, nn_l);
+    load3(in.normals, attr_row(in.nrm_index, c.slot[k]), n);
+    normalise3(n, nn);
+    const float intensity = dot3(nn, nn_l);
+    v[3 * k + 0] = sc.light_colour[0] * intensity;
+    v[3 * k + 1] = sc.light_colour[1] * intensity;
+    v[3 * k + 2] = sc.light_colour[2] * intensity;
+    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
+  }
+  static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                             float v[N]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) corner(in, sc, c, k, v);
+  }
+  static JR_DEV void scene_terms(const jr_scene&, float l[3]) { l[0] = l[1] = l[2] = 0.0f; }
+  static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
+                           const float*, const float b[3], bool front, float col[3], int) {
+    const float* uvv = v + 9;
+    float light[3];
+    bool pos = true;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      light[ch] = interp(b, v[ch], v[3 + ch], v[6 + ch]);
+      pos = pos && (light[ch] >= 0.0f);
+    }
+    const float uu = interp(b, uvv[0], uvv[2], uvv[4]);
+    const float vv = interp(b, uvv[1], uvv[3], uvv[5]);
+    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
+    const int32_t t = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    float tex[3];
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, t, tex);
+    col[0] = tex[0] * light[0];
+    col[1] = tex[1] * light[1];
+    col[2] = tex[2] * light[2];
+    return front && pos;
+  }
+};
+
+// shaders/phong.py:74-164: per corner the eye-space normal and the uv.
+template <>
+struct Varyings<JR_SHADER_PHONG> {
+  static constexpr int N = 15;
+  static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
+                            float* v) {
+    eye_normal(in, sc, c.slot[k], v + 3 * k);
+    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
+  }
+  static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                             float v[N]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) corner(in, sc, c, k, v);
+  }
+  static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
+    normalise3(sc.light_direction, l);
+  }
+  static JR_DEV bool pixel(const jr_inputs& in, const jr_scene& sc, const float v[N],
+                           const float ln[3], const float b[3], bool front, float col[3],
+                           int) {
+    const float* ne = v;
+    const float* uvv = v + 9;
+    float n[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) n[q] = interp(b, ne[q], ne[3 + q], ne[6 + q]);
+    const float uu = interp(b, uvv[0], uvv[2], uvv[4]);
+    const float vv = interp(b, uvv[1], uvv[3], uvv[5]);
+    const int32_t u = floor_mod(f2i_sat(floorf(uu)), sc.tex_w);
+    const int32_t t = floor_mod(f2i_sat(floorf(vv)), sc.tex_h);
+    float tex[3], nn[3];
+    tex_rgb(in.textures, sc.tex_base, sc.tex_h, sc.tex_c, u, t, tex);
+    normalise3(n, nn);
+    const float d = dot3(nn, ln);
+    float lc[3];
+    bool pos = true;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      lc[ch] = sc.light_colour[ch] * d;
+      pos = pos && (lc[ch] >= 0.0f);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) col[ch] = pos ? tex[ch] * lc[ch] : 0.0f;
+    return front;
+  }
+};
+
+// The shaders above, whole, for a pixel of the triangle with corners c.
+template <int SHADER>
+struct ShadeVaryings {
+  static JR_DEV bool run(const jr_inputs& in, const jr_scene& sc, const Corner& c,
+                         const float b[3], bool front, float col[3], int scene) {
+    using V = Varyings<SHADER>;
+    float v[V::N], l[3];
+    V::corners(in, sc, c, v);
+    V::scene_terms(sc, l);
+    return V::pixel(in, sc, v, l, b, front, col, scene);
+  }
+};
+template <>
+struct Shade<JR_SHADER_GOURAUD> : ShadeVaryings<JR_SHADER_GOURAUD> {};
+template <>
+struct Shade<JR_SHADER_GOURAUD_TEXTURE> : ShadeVaryings<JR_SHADER_GOURAUD_TEXTURE> {};
+template <>
+struct Shade<JR_SHADER_PHONG> : ShadeVaryings<JR_SHADER_PHONG> {};
+
 // shaders/phong_reflection.py:100-235: per corner the eye-space normal and
 // uv; the texture index of corner 0.
 template <>
@@ -380,16 +423,21 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
   static constexpr int N = 18;
   static constexpr int kTi = 15;     // texture index of corner 0
   static constexpr int kShape = 16;  // its texture_shape row (2 ints)
+  // corner k's entries of v; corner 0 also the triangle's texture index and shape
+  static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
+                            float* v) {
+    eye_normal(in, sc, c.slot[k], v + 3 * k);
+    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
+    if (k == 0) {
+      const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+      v[kTi] = __int_as_float(ti);
+      texture_shape(in, sc, ti, v + kShape, v + kShape + 1);
+    }
+  }
   static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                              float v[N]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], v + 3 * k);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
-    }
-    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
-    v[kTi] = __int_as_float(ti);
-    texture_shape(in, sc, ti, v + kShape, v + kShape + 1);
+    for (int k = 0; k < 3; ++k) corner(in, sc, c, k, v);
   }
   // per-scene terms of the pixel stage, computed once per tile
   static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
@@ -425,23 +473,27 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
   static constexpr int N = 30;
   static constexpr int kTi = 27;
   static constexpr int kShape = 28;
+  static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
+                            float* v) {
+    eye_normal(in, sc, c.slot[k], v + 3 * k);
+    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
+    float h[4];
+    position_clip(in, sc.shadow_world_to_clip, c.slot[k], h);
+    // normalise_homogeneous: every component divided by w
+    v[15 + 4 * k + 0] = h[0] / h[3];
+    v[15 + 4 * k + 1] = h[1] / h[3];
+    v[15 + 4 * k + 2] = h[2] / h[3];
+    v[15 + 4 * k + 3] = h[3] / h[3];
+    if (k == 0) {
+      const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
+      v[kTi] = __int_as_float(ti);
+      texture_shape(in, sc, ti, v + kShape, v + kShape + 1);
+    }
+  }
   static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                              float v[N]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      eye_normal(in, sc, c.slot[k], v + 3 * k);
-      load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
-      float h[4];
-      position_clip(in, sc.shadow_world_to_clip, c.slot[k], h);
-      // normalise_homogeneous: every component divided by w
-      v[15 + 4 * k + 0] = h[0] / h[3];
-      v[15 + 4 * k + 1] = h[1] / h[3];
-      v[15 + 4 * k + 2] = h[2] / h[3];
-      v[15 + 4 * k + 3] = h[3] / h[3];
-    }
-    const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
-    v[kTi] = __int_as_float(ti);
-    texture_shape(in, sc, ti, v + kShape, v + kShape + 1);
+    for (int k = 0; k < 3; ++k) corner(in, sc, c, k, v);
   }
   static JR_DEV void scene_terms(const jr_scene& sc, float l[3]) {
     normalise3(sc.light_dir_eye, l);

@@ -31,7 +31,9 @@ def test_library_exports_every_declared_function():
 
 def test_struct_layouts_match_the_binding():
     lib = _hip.lib()  # raises on a size mismatch
-    assert lib.jr_version() == 1
+    assert lib.jr_version() == _hip.ABI_VERSION
+    with open(HEADER) as f:
+        assert f"#define JR_ABI_VERSION {_hip.ABI_VERSION}\n" in f.read()
     for which, st in enumerate(_hip.STRUCTS):
         assert lib.jr_sizeof(which) == ctypes.sizeof(st), st.__name__
     assert lib.jr_sizeof(len(_hip.STRUCTS)) == -1

@@ -23,6 +23,7 @@ from jaxrenderer_amd.shaders import (
     GouraudTextureShader,
     PhongReflectionShadowTextureShader,
     PhongReflectionTextureShader,
+    PhongTextureDarbouxShader,
     PhongTextureShader,
 )
 from jaxrenderer_amd.types import LightSource
@@ -31,7 +32,8 @@ from tests import scenes, torchref
 pytestmark = pytest.mark.gpu
 
 SHADERS = [DepthShader, GouraudShader, GouraudTextureShader, PhongTextureShader,
-           PhongReflectionTextureShader, PhongReflectionShadowTextureShader]
+           PhongTextureDarbouxShader, PhongReflectionTextureShader,
+           PhongReflectionShadowTextureShader]
 W, H = 48, 40
 
 
@@ -56,7 +58,7 @@ def _to(x, dtype, dev, grad=True):
 
 
 GRAD_FIELDS = ("position", "normal", "colour", "texture", "specular_map", "light_dir_eye",
-               "ambient", "diffuse", "specular")
+               "ambient", "diffuse", "specular", "uv", "normal_map")
 
 
 def _leaves(cam, extra, dtype, dev):
@@ -94,9 +96,9 @@ def _ref_params(L, extra, shadow_cam):
         if f in L:
             P[k] = L[f]
     if hasattr(extra, "uv"):
-        P["uv"] = torch.as_tensor(np.asarray(extra.uv), dtype=torch.float64)
+        P["uv"] = L["uv"]
     if hasattr(extra, "normal_map"):
-        P["normal_map"] = torch.as_tensor(np.asarray(extra.normal_map), dtype=torch.float64)
+        P["normal_map"] = L["normal_map"]
         P["id_to_face"] = torch.as_tensor(np.asarray(extra.id_to_face), dtype=torch.int64)
         P["faces_indices"] = torch.as_tensor(np.asarray(extra.faces_indices), dtype=torch.int64)
     if hasattr(extra, "texture_shape"):

@@ -87,3 +87,24 @@ def test_batch_backward_without_face_table_matches_explicit_table():
         assert torch.isfinite(b).all(), k
         scale = max(float(b.norm()), 1e-6)
         assert float((a - b).norm()) <= 1e-5 * scale, k
+
+
+def test_fused_l2_backward_matches_explicit_cotangent():
+    """jr_backward_l2 (the canvas cotangent 2 (canvas - target) formed inside
+    the kernel) against jr_backward given that cotangent as a tensor; equal up
+    to the order of the atomic sums."""
+    br = _renderer(n=16, seed=9)
+    br.enable_backward()
+    out = br.render()
+    target = torch.rand_like(out["canvas"])
+    torch.cuda.synchronize()
+    got = {k: v.clone() for k, v in br.backward_l2(target).items()}
+    torch.cuda.synchronize()
+    want = br.backward(2.0 * (out["canvas"] - target))
+    torch.cuda.synchronize()
+    for k in want:
+        a, b = got[k].double(), want[k].double()
+        assert torch.isfinite(b).all(), k
+        scale = max(float(b.norm()), 1e-6)
+        assert float((a - b).norm()) <= 1e-5 * scale, k
+    assert float(want["d_scene"].norm()) > 0

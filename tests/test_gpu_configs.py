@@ -105,14 +105,15 @@ def _leaves(cam, extra, dtype, dev):
     def t(x):
         return torch.as_tensor(np.asarray(x), dtype=dtype, device=dev).clone().requires_grad_(True)
 
-    L = {k: t(getattr(extra, k)) for k in ("position", "normal", "colour", "texture")
-         if hasattr(extra, k)}
+    # phong_darboux's image depends on the uvs and the normal map continuously
+    keys = ("position", "normal", "colour", "texture") + (
+        ("uv", "normal_map") if hasattr(extra, "normal_map") else ())
+    L = {k: t(getattr(extra, k)) for k in keys if hasattr(extra, k)}
     L["light_direction"] = t(extra.light.direction)
     L["light_colour"] = t(extra.light.colour)
     for k in ("world_to_clip", "viewport", "world_to_eye_norm"):
         L[k] = t(getattr(cam, k))
-    ex = extra._replace(**{k: L[k] for k in ("position", "normal", "colour", "texture")
-                           if k in L},
+    ex = extra._replace(**{k: L[k] for k in keys if k in L},
                         light=LightSource(direction=L["light_direction"],
                                           colour=L["light_colour"]))
     cam2 = cam._replace(world_to_clip=L["world_to_clip"], viewport=L["viewport"],
@@ -128,7 +129,11 @@ def _ref_forward(shader, cam, faces, extra, pid, dtype):
          "light_dir": L["light_direction"], "light_col": L["light_colour"]}
     if "texture" in L:
         P["texture"] = L["texture"]
-        P["uv"] = torch.as_tensor(np.asarray(extra.uv), dtype=dtype)
+        P["uv"] = L["uv"] if "uv" in L else torch.as_tensor(np.asarray(extra.uv), dtype=dtype)
+    if "normal_map" in L:
+        P["normal_map"] = L["normal_map"]
+        P["id_to_face"] = torch.as_tensor(np.asarray(extra.id_to_face), dtype=torch.int64)
+        P["faces_indices"] = torch.as_tensor(np.asarray(extra.faces_indices), dtype=torch.int64)
     if "colour" in L:
         P["colour"] = L["colour"]
     S = pid.shape[0]
@@ -251,3 +256,66 @@ def test_darboux_head_512(tile, ppl):
                                              512, init_buffers=True)],
                          f"darboux-head/{tile}", tile=tile, pixels_per_lane=ppl)
     assert int((as_np(got["main"]["prim_id"]) >= 0).sum()) > 40_000
+
+
+def test_darboux_head_backward_against_float64():
+    """phong_darboux's backward at scale: the african head at 512x512 with its
+    512x512 diffuse and normal maps.  jr_backward's gradients of positions
+    (through visibility and the tangent frame's face corners), normals, uvs
+    (the tangent frame), texture, normal map, light and camera against torch
+    autograd of the float64 restatement with the GPU's primitive ids frozen.
+    Pixels at a float32 decision boundary (texel index, lit/unlit branch) or
+    whose tangent-frame matrix has a float64 condition number of 1e3 or more
+    carry no loss weight.
+
+    Two bars, as for the view workload: 1e-4 relative per tensor over pixels
+    of triangles whose M = [x y w] has a condition number below 300, and over
+    every pixel 1e-3 relative or no further from float64 than 1.5 times the
+    same formulas evaluated in float32."""
+    from jaxrenderer_amd.shaders import PhongTextureDarbouxShader as Sh
+    from tests import scenes
+
+    S = 512
+    print("configs: darboux head backward", flush=True)
+    cam, faces, extra = scenes.darboux_head(size=S)
+    dev = torch.device("cuda")
+    call = build_views_call([cam], Sh, faces, extra, S, S, init_buffers=True)
+    pid = _plan.run(call, prim_id=True)["prim_id"][0].cpu().numpy()
+    assert (pid >= 0).sum() > 40_000
+    L64, canvas64, z64, amb = _ref_forward(Sh, cam, faces, extra, pid, torch.float64)
+    L32c, canvas32, z32, _ = _ref_forward(Sh, cam, faces, extra, pid, torch.float32)
+    w2c = np.asarray(cam.world_to_clip, np.float64)
+    clip = np.concatenate([np.asarray(extra.position, np.float64),
+                           np.ones((len(extra.position), 1))], axis=1) @ w2c.T
+    cond = np.linalg.cond(clip[np.asarray(faces)][..., [0, 1, 3]])
+    pix_cond = np.where(pid >= 0, cond[np.maximum(pid, 0)], np.inf)
+    rng = np.random.default_rng(500)
+    wc_all = torch.as_tensor(rng.normal(size=(S, S, 3)))
+    wz_all = torch.as_tensor(rng.normal(size=(S, S)))
+    checked = 0
+    for label, mask in (("well-conditioned", (pid >= 0) & (pix_cond < 300)),
+                        ("all", pid >= 0)):
+        keep = torch.as_tensor(mask) & ~amb
+        wc, wz = wc_all * keep[..., None], wz_all * keep
+        g64 = _grads(L64, canvas64, z64, wc, wz)
+        g32c = _grads(L32c, canvas32, z32, wc, wz)
+        L32, cam32, ex32 = _leaves(cam, extra, torch.float32, dev)
+        zb, cv = render_differentiable(cam32, Sh, faces, ex32, S, S)
+        gpu = _grads(L32, cv, zb, wc, wz)
+        for n in g64:
+            ref, got = g64[n], gpu[n]
+            scale = float(ref.norm())
+            err = float((got - ref).norm())
+            if scale < 1e-8:
+                assert err < 1e-6, f"{label} {n}: expected ~0 gradient, got norm {err}"
+                continue
+            checked += 1
+            if label == "all":
+                f32_err = float((g32c[n] - ref).norm())
+                assert err <= max(1e-3 * scale, 1.5 * f32_err), \
+                    f"{label} {n}: relative error {err / scale:.3e}, float32 restatement " \
+                    f"{f32_err / scale:.3e}"
+            else:
+                assert err <= 1e-4 * scale, \
+                    f"{label} {n}: relative error {err / scale:.3e} (|ref|={scale:.3e})"
+    assert checked >= 14

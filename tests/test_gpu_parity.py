@@ -28,7 +28,7 @@ def _check(make_calls, label, **kw):
 
 def test_library_loads_on_gpu():
     from jaxrenderer_amd import _hip
-    assert _hip.lib().jr_version() == 1
+    assert _hip.lib().jr_version() == _hip.ABI_VERSION == 2
     assert torch.cuda.is_available()
 
 
