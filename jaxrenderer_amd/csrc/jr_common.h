@@ -147,17 +147,41 @@ struct TriSetup {
 };
 
 // Clip coordinates of face row t's three slots (the vertex stage's positions).
+// The uniform choices (face table, clip input, position index) are made once,
+// outside the per-corner code, so the three corners' loads of each level are
+// issued together: two round trips to memory instead of six.
 JR_DEV void tri_corners(const jr_inputs& in, const jr_scene& sc, int32_t t, float clip[3][4]) {
+  int32_t slot[3];
+  if (in.faces) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int32_t slot = face_slot(in, t, k);
-    if (in.clip) {
+    for (int k = 0; k < 3; ++k) slot[k] = in.faces[3 * (int64_t)t + k];
+  } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) clip[k][q] = in.clip[4 * (int64_t)slot + q];
-    } else {
-      position_clip(in, sc.world_to_clip, slot, clip[k]);
-    }
+    for (int k = 0; k < 3; ++k) slot[k] = 3 * t + k;
   }
+  if (in.clip) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) clip[k][q] = in.clip[4 * (int64_t)slot[k] + q];
+    return;
+  }
+  int32_t row[3];
+  if (in.pos_index) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) row[k] = in.pos_index[slot[k]];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) row[k] = slot[k];
+  }
+  float p[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) load3(in.positions, row[k], p[k]);
+  float W[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) W[i] = sc.world_to_clip[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) apply4(W, p[k][0], p[k][1], p[k][2], 1.0f, clip[k]);
 }
 
 // The [x, y, w] matrix of the corners and its determinant.

@@ -426,7 +426,13 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
       s = scene_of_wave(scenes, f.n_scenes, wave_scene, t, in.n_tris);
     if (t < in.n_tris) {
       const jr_scene& sc = scenes[s];
-      tri_corners(in, sc, t, clip);
+      // a wavefront of one scene (nearly every one) reads the camera matrix
+      // with scalar loads
+      const int32_t s_u = __builtin_amdgcn_readfirstlane(s);
+      if (__ballot(s != s_u) == 0)
+        tri_corners(in, scenes[s_u], t, clip);
+      else
+        tri_corners(in, sc, t, clip);
       float M[9];
       det = tri_det(clip, M);
       const bool keep = fabsf(det) > 1e-6f, front = det >= 0.0f;
