@@ -615,13 +615,16 @@ JR_DEV void stage_add(int32_t* tab, float (*val)[NC], float* __restrict__ d, int
 }
 
 // Adds the staged sums to d (n channels per entry) and leaves the table free.
+// Lanes take consecutive (entry, channel) pairs, so the channels of one entry
+// are added by neighbouring lanes of one instruction (same cache line).
 template <int CAP, int NC>
 JR_DEV void flush_adds(int32_t* tab, float (*val)[NC], float* __restrict__ d, int32_t n) {
-  for (int e = threadIdx.x; e < CAP; e += blockDim.x) {
+  if (!d) return;
+  for (int idx = threadIdx.x; idx < CAP * n; idx += blockDim.x) {
+    const int e = n == 3 ? idx / 3 : (n == 1 ? idx : idx / n);
+    const int c = idx - e * n;
     const int32_t at = tab[e];
-    if (at >= 0 && d)
-      for (int c = 0; c < n; ++c)
-        if (val[e][c] != 0.0f) atomicAdd(d + at + c, val[e][c]);
+    if (at >= 0 && val[e][c] != 0.0f) atomicAdd(d + at + c, val[e][c]);
   }
 }
 
@@ -736,11 +739,14 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
         r[4 * k + 2] = c.z;
         r[4 * k + 3] = c.w;
       }
-      if constexpr (NV > 0) {
-        float v[NV > 0 ? NV : 1];
-        Varyings<SHADER>::corners(a.in, sc, corner_slots(a.in, t), v);
-        for (int k = 0; k < NV; ++k) r[12 + k] = v[k];
-      }
+    }
+  }
+  // the corner stage: one lane per corner of each listed triangle
+  if constexpr (NV > 0) {
+    for (int idx = lane; idx < 3 * kBwdCap; idx += 64) {
+      const int e = idx / 3, k = idx - 3 * e;
+      const int32_t t = tab[e];
+      if (t >= 0) Varyings<SHADER>::corner(a.in, sc, corner_slots(a.in, t), k, trec + e * NS + 12);
     }
   }
   __syncthreads();
@@ -919,15 +925,15 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   flush_adds<kTexCap, 3>(tex_tab, tex_val, a.gr.d_textures, sc.tex_c > 1 ? 3 : 1);
   flush_adds<kSpecCap, 1>(spec_tab, spec_val, a.gr.d_specular_maps, 1);
   if constexpr (DARBOUX) flush_adds<kNmapCap, 3>(nmap_tab, nmap_val, a.gr.d_normal_maps, 3);
-  if (lane < kBwdCap) {
-    const int32_t t = tab[lane];
-    if (t >= 0) {
-      a.touched[t] = s + 1;
-      for (int k = 0; k < ACC; ++k) {
-        const float v = acc[lane][k];
-        if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * ACC + k, v);
-      }
-    }
+  if (lane < kBwdCap && tab[lane] >= 0) a.touched[tab[lane]] = s + 1;
+  // lanes take consecutive accumulators of a triangle's row: the addresses of
+  // one atomic instruction fall in few cache lines
+  for (int idx = lane; idx < kBwdCap * ACC; idx += 64) {
+    const int e = idx / ACC, k = idx - e * ACC;
+    const int32_t t = tab[e];
+    if (t < 0) continue;
+    const float v = acc[e][k];
+    if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * ACC + k, v);
   }
   // per-scene parameters: one reduction per tile
   if (a.gr.d_scene) {

@@ -126,12 +126,17 @@ JR_DEV int32_t face_slot(const jr_inputs& in, int32_t t, int k) {
   return in.faces ? in.faces[3 * (int64_t)t + k] : 3 * t + k;
 }
 
-// The slots of face row w's corners.
+// The slots of face row w's corners (the face table's test made once, so its
+// three loads issue together).
 JR_DEV Corner corner_slots(const jr_inputs& in, int32_t w) {
   Corner c;
-  c.slot[0] = face_slot(in, w, 0);
-  c.slot[1] = face_slot(in, w, 1);
-  c.slot[2] = face_slot(in, w, 2);
+  if (in.faces) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c.slot[k] = in.faces[3 * (int64_t)w + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c.slot[k] = 3 * w + k;
+  }
   return c;
 }
 

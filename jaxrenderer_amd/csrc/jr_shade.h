@@ -67,13 +67,37 @@ JR_DEV void gouraud_vertex(const jr_inputs& in, const jr_scene& sc, int32_t slot
 }
 
 // Eye-space normal of the phong family's vertex stage
-// (shaders/phong.py:74-104): apply_vec(normalise(n), world_to_eye_norm).
-JR_DEV void eye_normal(const jr_inputs& in, const jr_scene& sc, int32_t slot,
-                       float out[3]) {
-  float n[3], nn[3];
-  load3(in.normals, attr_row(in.nrm_index, slot), n);
+// (shaders/phong.py:74-104): apply_vec(normalise(n), world_to_eye_norm), of
+// a loaded normal n.
+JR_DEV void eye_normal_of(const float n[3], const jr_scene& sc, float out[3]) {
+  float nn[3];
   normalise3(n, nn);
   apply_vec(sc.world_to_eye_norm, nn, out);
+}
+JR_DEV void eye_normal(const jr_inputs& in, const jr_scene& sc, int32_t slot,
+                       float out[3]) {
+  float n[3];
+  load3(in.normals, attr_row(in.nrm_index, slot), n);
+  eye_normal_of(n, sc, out);
+}
+
+// A slot's attribute rows; with every index array present (the usual case)
+// the three index loads issue together.
+struct SlotRows {
+  int32_t pos, nrm, uv;
+};
+JR_DEV SlotRows slot_rows(const jr_inputs& in, int32_t slot) {
+  SlotRows r;
+  if (in.pos_index && in.nrm_index && in.uv_index) {
+    r.pos = in.pos_index[slot];
+    r.nrm = in.nrm_index[slot];
+    r.uv = in.uv_index[slot];
+  } else {
+    r.pos = attr_row(in.pos_index, slot);
+    r.nrm = attr_row(in.nrm_index, slot);
+    r.uv = attr_row(in.uv_index, slot);
+  }
+  return r;
 }
 
 JR_DEV void position_clip(const jr_inputs& in, const float* M, int32_t slot,
@@ -312,15 +336,16 @@ struct Varyings<JR_SHADER_GOURAUD_TEXTURE> {
   static constexpr int N = 15;
   static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
                             float* v) {
+    const SlotRows r = slot_rows(in, c.slot[k]);
     float nn_l[3], n[3], nn[3];
+    load3(in.normals, r.nrm, n);
+    load2(in.uvs, r.uv, v + 9 + 2 * k);
     normalise3(sc.light_direction, nn_l);
-    load3(in.normals, attr_row(in.nrm_index, c.slot[k]), n);
     normalise3(n, nn);
     const float intensity = dot3(nn, nn_l);
     v[3 * k + 0] = sc.light_colour[0] * intensity;
     v[3 * k + 1] = sc.light_colour[1] * intensity;
     v[3 * k + 2] = sc.light_colour[2] * intensity;
-    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
   }
   static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                              float v[N]) {
@@ -357,8 +382,11 @@ struct Varyings<JR_SHADER_PHONG> {
   static constexpr int N = 15;
   static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
                             float* v) {
-    eye_normal(in, sc, c.slot[k], v + 3 * k);
-    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
+    const SlotRows r = slot_rows(in, c.slot[k]);
+    float n[3];
+    load3(in.normals, r.nrm, n);
+    load2(in.uvs, r.uv, v + 9 + 2 * k);
+    eye_normal_of(n, sc, v + 3 * k);
   }
   static JR_DEV void corners(const jr_inputs& in, const jr_scene& sc, const Corner& c,
                              float v[N]) {
@@ -426,8 +454,11 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION> {
   // corner k's entries of v; corner 0 also the triangle's texture index and shape
   static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
                             float* v) {
-    eye_normal(in, sc, c.slot[k], v + 3 * k);
-    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
+    const SlotRows r = slot_rows(in, c.slot[k]);
+    float n[3];
+    load3(in.normals, r.nrm, n);
+    load2(in.uvs, r.uv, v + 9 + 2 * k);
+    eye_normal_of(n, sc, v + 3 * k);
     if (k == 0) {
       const int32_t ti = in.tex_index[attr_row(in.tex_index_index, c.slot[0])];
       v[kTi] = __int_as_float(ti);
@@ -475,10 +506,13 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
   static constexpr int kShape = 28;
   static JR_DEV void corner(const jr_inputs& in, const jr_scene& sc, const Corner& c, int k,
                             float* v) {
-    eye_normal(in, sc, c.slot[k], v + 3 * k);
-    load2(in.uvs, attr_row(in.uv_index, c.slot[k]), v + 9 + 2 * k);
-    float h[4];
-    position_clip(in, sc.shadow_world_to_clip, c.slot[k], h);
+    const SlotRows r = slot_rows(in, c.slot[k]);
+    float n[3], p[3], h[4];
+    load3(in.normals, r.nrm, n);
+    load3(in.positions, r.pos, p);
+    load2(in.uvs, r.uv, v + 9 + 2 * k);
+    eye_normal_of(n, sc, v + 3 * k);
+    apply4(sc.shadow_world_to_clip, p[0], p[1], p[2], 1.0f, h);
     // normalise_homogeneous: every component divided by w
     v[15 + 4 * k + 0] = h[0] / h[3];
     v[15 + 4 * k + 1] = h[1] / h[3];
