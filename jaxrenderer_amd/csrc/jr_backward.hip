@@ -239,13 +239,12 @@ struct ShadeBwd<JR_SHADER_PHONG> : CornerBwd<JR_SHADER_PHONG> {
       dd += dlc[q] * sc.light_colour[q];
     }
     add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, t, dt);
-    float dnn[3], dln[3], dn[3], dL[3];
+    float dnn[3], dln[3], dn[3];
     for (int q = 0; q < 3; ++q) {
       dnn[q] = dd * ln[q];
       dln[q] = dd * nn[q];
+      g.sg[S_LIGHT_DIR + q] += dln[q];  // normalise's backward applied per tile
     }
-    normalise_bwd(sc.light_direction, dln, dL);
-    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR + q] += dL[q];
     normalise_bwd(n, dnn, dn);
     for (int k = 0; k < 3; ++k) {
       g.db[k] = dot3(dn, ne + 3 * k);
@@ -349,13 +348,12 @@ struct ShadeBwd<JR_SHADER_PHONG_DARBOUX> {
       dd += dlc * sc.light_colour[q];
     }
     add_tex(g, sc.tex_base, sc.tex_h, sc.tex_c, u, v, dt);
-    float dn2[3], dln[3], dL[3], dbn[3];
+    float dn2[3], dln[3], dbn[3];
     for (int q = 0; q < 3; ++q) {
       dn2[q] = dd * ln[q];
       dln[q] = dd * n2[q];
+      g.sg[S_LIGHT_DIR + q] += dln[q];  // normalise's backward applied per tile
     }
-    normalise_bwd(sc.light_direction, dln, dL);
-    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR + q] += dL[q];
     normalise_bwd(bn, dn2, dbn);
     // bn = B @ nm with B's columns in_, jn, n
     float din[3], djn[3], dn[3];
@@ -520,10 +518,10 @@ struct ReflectionBwd {
       dnh[q] += dd * lh[q];
       dlh[q] += dd * nh[q];
     }
-    float dn[3], dle[3];
+    float dn[3];
     normalise_bwd(n, dnh, dn);
-    normalise_bwd(sc.light_dir_eye, dlh, dle);
-    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR_EYE + q] += dle[q];
+    // normalise's backward is linear in its cotangent: applied once per tile
+    for (int q = 0; q < 3; ++q) g.sg[S_LIGHT_DIR_EYE + q] += dlh[q];
     for (int k = 0; k < 3; ++k) {
       g.db[k] = dot3(dn, nev + 3 * k);
       for (int q = 0; q < 3; ++q) g.dvar[k][q] = b[k] * dn[q];
@@ -694,6 +692,9 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
   const int32_t tx = udiv31(tile, a.ty_m, a.ty_sh), ty = tile - tx * a.tiles_y;
   const jr_scene& sc = a.scenes[s];
   const float* vp = sc.viewport;
+  // reciprocals for the viewport gradients only (x and y themselves are the
+  // forward's quotients)
+  const float ivp0 = 1.0f / vp[0], ivp5 = 1.0f / vp[5];
   auto pixel_of = [&](int q, int32_t* i_out, int32_t* j_out) -> int32_t {
     const int32_t i = tx * kBwdTile + 2 * (lane >> 3) + (q >> 1);
     const int32_t j = ty * kBwdTile + 2 * (lane & 7) + (q & 1);
@@ -870,10 +871,10 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
         dy += m1[k] * de[k];
       }
       // x = (i - vp03) / vp00, y = (j - vp13) / vp11
-      g.sg[S_VP03] += -dx / vp[0];
-      g.sg[S_VP00] += -dx * x / vp[0];
-      g.sg[S_VP13] += -dy / vp[5];
-      g.sg[S_VP11] += -dy * y / vp[5];
+      g.sg[S_VP03] += -dx * ivp0;
+      g.sg[S_VP00] += -dx * x * ivp0;
+      g.sg[S_VP13] += -dy * ivp5;
+      g.sg[S_VP11] += -dy * y * ivp5;
       for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) vals[A_VAR + 3 * k + c] = g.dvar[k][c];
       if constexpr (DARBOUX) {
@@ -935,20 +936,33 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
     const float v = acc[e][k];
     if (v != 0.0f) atomicAdd(a.tacc + (int64_t)t * ACC + k, v);
   }
-  // per-scene parameters: one reduction per tile
+  // per-scene parameters: one reduction per tile.  The pixels summed the
+  // cotangents of normalise(light_direction) and normalise(light_dir_eye);
+  // normalise's backward is linear in them, so it is applied to the sums.
   if (a.gr.d_scene) {
-    for (int q = 0; q < S_COUNT; ++q) {
-      const float v = wave_sum(g.sg[q]);
-      if (lane == 0 && v != 0.0f)
-        atomicAdd(a.gr.d_scene + (int64_t)s * JR_DSCENE_FLOATS + kSceneSlot[q], v);
-    }
+    float tot[S_COUNT];
+    for (int q = 0; q < S_COUNT; ++q) tot[q] = wave_sum(g.sg[q]);
+    auto through_normalise = [&](const float* v, int at) {
+      if (tot[at] == 0.0f && tot[at + 1] == 0.0f && tot[at + 2] == 0.0f) return;
+      float d[3];
+      normalise_bwd(v, tot + at, d);
+      for (int q = 0; q < 3; ++q) tot[at + q] = d[q];
+    };
+    through_normalise(sc.light_direction, S_LIGHT_DIR);
+    through_normalise(sc.light_dir_eye, S_LIGHT_DIR_EYE);
+    for (int q = 0; q < S_COUNT; ++q)
+      if (lane == 0 && tot[q] != 0.0f)
+        atomicAdd(a.gr.d_scene + (int64_t)s * JR_DSCENE_FLOATS + kSceneSlot[q], tot[q]);
   }
 }
 
-// vertex-stage backward of one slot for its varying cotangent dv
+// vertex-stage backward of one slot for its varying cotangent dv: the
+// gradient dn of the slot's normal row *nrow_out (and for gouraud dc of its
+// colour row *crow_out; rows stay -1 when there is nothing to add)
 template <int SHADER>
 JR_DEV void vertex_bwd(const BwdArgs& a, const jr_scene& sc, int32_t slot, const float dv[3],
-                       float* dW, float* dE, float* dLd, float* dLc) {
+                       float* dW, float* dE, float* dLd, float* dLc, int64_t* nrow_out,
+                       float dn[3], int64_t* crow_out, float dc[3]) {
   if (SHADER == JR_SHADER_DEPTH) return;
   if (dv[0] == 0.0f && dv[1] == 0.0f && dv[2] == 0.0f) return;
   const jr_inputs& in = a.in;
@@ -966,9 +980,9 @@ JR_DEV void vertex_bwd(const BwdArgs& a, const jr_scene& sc, int32_t slot, const
       const int32_t crow = attr_row(in.col_index, slot);
       float col[3];
       load3(in.colours, crow, col);
+      *crow_out = crow;
       for (int q = 0; q < 3; ++q) {
-        if (a.gr.d_colours)
-          atomicAdd(a.gr.d_colours + 3 * (int64_t)crow + q, dv[q] * sc.light_colour[q] * it);
+        dc[q] = dv[q] * sc.light_colour[q] * it;
         dLc[q] += dv[q] * col[q] * it;
         dI += dv[q] * col[q] * sc.light_colour[q];
       }
@@ -1005,37 +1019,65 @@ JR_DEV void vertex_bwd(const BwdArgs& a, const jr_scene& sc, int32_t slot, const
     normalise_bwd(nn, du2, du1);
     for (int q = 0; q < 3; ++q) dnn[q] = du1[q];
   }
-  float dn[3];
   normalise_bwd(n, dnn, dn);
-  if (a.gr.d_normals)
-    for (int q = 0; q < 3; ++q) atomicAdd(a.gr.d_normals + 3 * (int64_t)nrow + q, dn[q]);
+  *nrow_out = nrow;
+}
+
+// Adds each lane's 3-float row g at d + 3 row (row < 0: nothing) with
+// neighbouring lanes on the three floats of one row, so the adds of an atomic
+// instruction share cache lines.  W: the wavefront's LDS staging; call with
+// every lane of the wavefront.
+struct WaveRows {
+  int64_t at[64];
+  float v[64][3];
+};
+JR_DEV void wave_row_adds(WaveRows& W, float* __restrict__ d, int64_t row, const float g[3]) {
+  const int lane = threadIdx.x & 63;
+  W.at[lane] = row;
+  W.v[lane][0] = g[0];
+  W.v[lane][1] = g[1];
+  W.v[lane][2] = g[2];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int it = 0; it < 3; ++it) {
+    const int idx = lane + 64 * it;
+    const int e = idx / 3, c = idx - 3 * e;
+    const int64_t r = W.at[e];
+    const float x = W.v[e][c];
+    if (r >= 0 && x != 0.0f) atomicAdd(d + 3 * r + c, x);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Triangle t (-1: none) of the calling lane; the accumulators are read and
 // cleared only when `touched`.  Every lane computes its scene, so that a
 // wavefront of one scene reduces its camera and light gradients once.
 template <int SHADER>
-JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
+JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched, WaveRows& W) {
   float dW[16], dE[16], dLd[3], dLc[3];
   for (int q = 0; q < 16; ++q) dW[q] = dE[q] = 0.0f;
   for (int q = 0; q < 3; ++q) dLd[q] = dLc[q] = 0.0f;
   if (__ballot(touched) == 0) return;  // nothing to add for this wavefront
   if (!touched) s = -1;
   constexpr int ACC = AccOf<SHADER>::N;
+  float v[ACC];
+  bool live = false;
+  TriSetup ts;
+  float dM[9];
   if (touched) {
     float* A = a.tacc + (int64_t)t * ACC;
-    bool any = false;
-    float v[ACC];
     for (int q = 0; q < ACC; ++q) {
       v[q] = A[q];
       A[q] = 0.0f;
-      any = any || (v[q] != 0.0f);
+      live = live || (v[q] != 0.0f);
     }
     a.touched[t] = 0;
-    if (any) {
-      const jr_scene& sc = a.scenes[s];
-      TriSetup ts;
-      tri_setup(a.in, sc, t, ts);
+    if (live) {
+      tri_setup(a.in, a.scenes[s], t, ts);
       // X = M^-1, G = dL/dX: dL/dM = -X^T G X^T
       const float* X = ts.Mi;
       float XtG[9];
@@ -1045,31 +1087,47 @@ JR_DEV void bwd_tri(const BwdArgs& a, int32_t t, int32_t s, bool touched) {
           for (int r = 0; r < 3; ++r) s2 += X[3 * r + p] * v[A_MINV + 3 * r + k];
           XtG[3 * p + k] = s2;
         }
-      float dM[9];
       for (int p = 0; p < 3; ++p)
         for (int q = 0; q < 3; ++q) {
           float s2 = 0.0f;
           for (int k = 0; k < 3; ++k) s2 += XtG[3 * p + k] * X[3 * q + k];
           dM[3 * p + q] = -s2;
         }
-      for (int k = 0; k < 3; ++k) {
-        const int32_t slot = face_slot(a.in, t, k);
-        const float dclip[4] = {dM[3 * k + 0], dM[3 * k + 1], v[A_Z + k], dM[3 * k + 2]};
-        const int32_t prow = attr_row(a.in.pos_index, slot);
-        float p[3];
-        load3(a.in.positions, prow, p);
-        const float ph[4] = {p[0], p[1], p[2], 1.0f};
-        for (int jj = 0; jj < 4; ++jj) {
-          float s2 = 0.0f;
-          for (int ii = 0; ii < 4; ++ii) {
-            s2 += sc.world_to_clip[4 * ii + jj] * dclip[ii];
-            dW[4 * ii + jj] += dclip[ii] * ph[jj];
-          }
-          if (jj < 3 && a.gr.d_positions) atomicAdd(a.gr.d_positions + 3 * (int64_t)prow + jj, s2);
+    }
+  }
+  // corner by corner, the lanes' position / normal / colour rows added
+  // together (wave_row_adds), every lane taking part
+  for (int k = 0; k < 3; ++k) {
+    int64_t prow = -1, nrow = -1, crow = -1;
+    float dp[3] = {0.0f, 0.0f, 0.0f}, dn[3] = {0.0f, 0.0f, 0.0f}, dc[3] = {0.0f, 0.0f, 0.0f};
+    if (live) {
+      const jr_scene& sc = a.scenes[s];
+      const int32_t slot = face_slot(a.in, t, k);
+      const float dclip[4] = {dM[3 * k + 0], dM[3 * k + 1], v[A_Z + k], dM[3 * k + 2]};
+      prow = attr_row(a.in.pos_index, slot);
+      float p[3];
+      load3(a.in.positions, prow, p);
+      const float ph[4] = {p[0], p[1], p[2], 1.0f};
+      for (int jj = 0; jj < 4; ++jj) {
+        float s2 = 0.0f;
+        for (int ii = 0; ii < 4; ++ii) {
+          s2 += sc.world_to_clip[4 * ii + jj] * dclip[ii];
+          dW[4 * ii + jj] += dclip[ii] * ph[jj];
         }
-        const float dv[3] = {v[A_VAR + 3 * k], v[A_VAR + 3 * k + 1], v[A_VAR + 3 * k + 2]};
-        vertex_bwd<SHADER>(a, sc, slot, dv, dW, dE, dLd, dLc);
+        if (jj < 3) dp[jj] = s2;
       }
+      const float dv[3] = {v[A_VAR + 3 * k], v[A_VAR + 3 * k + 1], v[A_VAR + 3 * k + 2]};
+      vertex_bwd<SHADER>(a, sc, slot, dv, dW, dE, dLd, dLc, &nrow, dn, &crow, dc);
+    }
+    if (a.gr.d_positions) wave_row_adds(W, a.gr.d_positions, prow, dp);
+    if constexpr (SHADER != JR_SHADER_DEPTH)
+      if (a.gr.d_normals) wave_row_adds(W, a.gr.d_normals, nrow, dn);
+    if constexpr (SHADER == JR_SHADER_GOURAUD)
+      if (a.gr.d_colours) wave_row_adds(W, a.gr.d_colours, crow, dc);
+  }
+  if (live) {
+    {
+      const jr_scene& sc = a.scenes[s];
       if constexpr (SHADER == JR_SHADER_PHONG_DARBOUX) {
         // the face of slot 0 (id_to_face) whose NDC corners and uvs built the
         // tangent frame: t_m = to_cartesian(h_m), h_m = world_to_clip @ [p_m, 1]
@@ -1166,6 +1224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
 void k_bwd_tris(BwdArgs a) {
   __shared__ int2 items[256];
   __shared__ int32_t wave_n[4];
+  __shared__ WaveRows rows[4];
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const int32_t flag = t < a.in.n_tris ? a.touched[t] : 0;
   const uint64_t bal = __ballot(flag != 0);
@@ -1179,7 +1238,7 @@ void k_bwd_tris(BwdArgs a) {
   const int32_t n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
   const bool has = (int32_t)threadIdx.x < n;
   const int2 it = has ? items[threadIdx.x] : make_int2(-1, -1);
-  bwd_tri<SHADER>(a, it.x, it.y, has);
+  bwd_tri<SHADER>(a, it.x, it.y, has, rows[threadIdx.x >> 6]);
 }
 
 // Reads the workspace header: if it does not describe this workspace (new,
