@@ -604,8 +604,9 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
       if (item >= total) continue;
       const int4 box = sm.box[o];
       const int32_t q = item - sm.off[o];
-      // q / h for q < 64, h <= 64: a float estimate, corrected to the exact quotient
-      int32_t di = (int32_t)((float)q / (float)box.z);
+      // q / h for q < 64, h <= 64: a float estimate (the hardware reciprocal,
+      // within a few ulps), corrected to the exact quotient
+      int32_t di = (int32_t)((float)q * __builtin_amdgcn_rcpf((float)box.z));
       if (di * box.z > q) --di;
       if ((di + 1) * box.z <= q) ++di;
       const int32_t i = box.x + di, j = box.y + (q - di * box.z);
