@@ -183,9 +183,9 @@ enum {
  * small-triangle keys and tile lists, with the same per-texel arithmetic,
  * chooser, fallback and offset as that pass's raster stage -- so the shadow
  * pass runs only its binning stages (jr_frame.part = JR_PART_BIN).  Each depth
- * found is also written to texels[scene][x][y], so the map is right at every
- * texel a pixel read: jr_backward, given shadow_maps = texels, reads the same
- * values.  The shadow pass must be a depth-only pass whose small-triangle keys
+ * found is also written to texels[scene][x][y] (unless texels is NULL), so
+ * the map is right at every texel a pixel read: jr_backward, given
+ * shadow_maps = texels, reads the same values.  The shadow pass must be a depth-only pass whose small-triangle keys
  * stay in its zbuffer: JR_SHADER_DEPTH, init_buffers = 1, apply_offset = 1, a
  * depth_offset other than -0, no prim_id / bary outputs; and its JR_PART_BIN
  * call must have run on `workspace` and `keys` (stream ordered before this
@@ -196,7 +196,8 @@ typedef struct jr_shadow_source {
   const void* workspace;    /* its workspace */
   int64_t workspace_bytes;
   const float* keys;        /* its zbuffer after JR_PART_BIN: per-texel 32-bit keys */
-  float* texels;            /* [n_scenes][frame->width][frame->height], written */
+  float* texels;            /* [n_scenes][frame->width][frame->height], written;
+                               NULL: not kept (a forward-only render) */
   int32_t n_tris;           /* its triangle count */
   int32_t pad;
 } jr_shadow_source;

@@ -289,12 +289,13 @@ class ShadowSource:
     """A shadow pass (depth shader, offset, no id outputs) whose binning stages
     ran on `workspace` with `keys` as its zbuffer, for a phong_reflection_shadow
     pass that computes the texel depths it reads (jr_shadow_source).  `texels`
-    [n_scenes, width, height] receives every depth so computed."""
+    [n_scenes, width, height] receives every depth so computed (None: they are
+    not kept)."""
     frame: _hip.JrFrame
     scenes: torch.Tensor
     workspace: torch.Tensor
     keys: torch.Tensor
-    texels: torch.Tensor
+    texels: Optional[torch.Tensor]
     n_tris: int
 
     def struct(self, dev: torch.device):
@@ -306,7 +307,8 @@ class ShadowSource:
         s.keys = _ptr(self.keys, "shadow keys", torch.float32, dev)
         s.texels = _ptr(self.texels, "shadow texels", torch.float32, dev)
         n, w, h = self.frame.n_scenes, self.frame.width, self.frame.height
-        if tuple(self.texels.shape) != (n, w, h) or tuple(self.keys.shape) != (n, w, h):
+        if tuple(self.keys.shape) != (n, w, h) or (
+                self.texels is not None and tuple(self.texels.shape) != (n, w, h)):
             raise ValueError(f"shadow keys and texels must be [{n}, {w}, {h}]")
         s.n_tris = int(self.n_tris)
         return s, (self.frame,)

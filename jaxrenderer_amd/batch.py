@@ -247,7 +247,7 @@ class BatchRenderer:
 
     def __init__(self, calls: List[RasterCall], *, tile=(8, 8), shadow_tile=None,
                  persistent: bool = True, want_ids: bool = False, display: bool = False,
-                 shadow_on_demand=False):
+                 shadow_on_demand=False, keep_shadow_texels: bool = False):
         """`display`: also write the main pass's uint8 display images
         [B, H, W, 3] (clamp to [0, 1], x 255, truncate, transpose_for_display).
 
@@ -255,8 +255,10 @@ class BatchRenderer:
         the main pass computes the depth of each shadow-map texel it reads
         from that binning (jr_shadow_source), with the shadow raster's own
         arithmetic, so the image is the same bit for bit.  The depths it
-        computed are kept in `shadow_texels` (the backward reads them there);
-        the other texels are not computed, so no whole shadow map is output.
+        computed are kept in `shadow_texels` when `keep_shadow_texels` is set
+        or once enable_backward() has been called (the backward reads them
+        there); a forward-only renderer skips those stores.  The other texels
+        are not computed, so no whole shadow map is output.
         "auto": on when the shadow map has at least ON_DEMAND_RATIO times as
         many texels as the main pass has pixels (then most texels are never
         read, and computing only the read ones is the cheaper way)."""
@@ -273,6 +275,7 @@ class BatchRenderer:
             shadow_on_demand = (sc.width * sc.height >=
                                 ON_DEMAND_RATIO * main.call.width * main.call.height)
         self.shadow_on_demand = bool(shadow_on_demand) and main.call.shadow_from is not None
+        self.keep_shadow_texels = bool(keep_shadow_texels)
         self.shadow_texels: Optional[torch.Tensor] = None
         if self.shadow_on_demand:
             sc = self.passes[main.call.shadow_from].call
@@ -297,7 +300,8 @@ class BatchRenderer:
         sp = self.passes[self.passes["main"].call.shadow_from]
         return _raster.ShadowSource(frame=sp.frame(), scenes=sp.scenes,
                                     workspace=sp.workspace, keys=sp.zbuffer,
-                                    texels=self.shadow_texels, n_tris=sp.call.n_tris)
+                                    texels=self.shadow_texels if self.keep_shadow_texels
+                                    else None, n_tris=sp.call.n_tris)
 
     def _launch(self, tag: str, part: int = _hip.PART_ALL) -> None:
         """One pass, as a render runs it: with shadows on demand the shadow
@@ -343,7 +347,8 @@ class BatchRenderer:
         if main.display is not None:
             out["display"] = main.display
         if self.shadow_on_demand:
-            out["shadow_texels"] = self.shadow_texels
+            if self.keep_shadow_texels:
+                out["shadow_texels"] = self.shadow_texels
         elif "shadow" in self.passes:
             out["shadow_map"] = self.passes["shadow"].zbuffer
         return out
@@ -403,6 +408,8 @@ class BatchRenderer:
         if main.prim_id is None:
             main.prim_id = torch.empty((c.n_scenes, c.width, c.height), dtype=torch.int32,
                                        device=dev)
+        # the backward reads the shadow depths the forward computed on demand
+        self.keep_shadow_texels = True
         self.grads = {}
         for name, key in (("d_positions", "positions"), ("d_normals", "normals"),
                           ("d_colours", "colours"), ("d_textures", "textures"),

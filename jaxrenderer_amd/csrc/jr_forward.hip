@@ -978,7 +978,7 @@ JR_DEV float shadow_depth_on_demand(const ShadowLookup& L, int32_t s, float x, f
       d = L.zdef + L.depth_offset;
     }
   }
-  L.texels[pix] = d;
+  if (L.texels) L.texels[pix] = d;
   return d;
 }
 
@@ -1667,9 +1667,8 @@ int32_t check_shadow_source(const jr_frame* fr, const jr_inputs* in) {
   const jr_shadow_source* src = in->shadow_source;
   if (fr->shader != JR_SHADER_PHONG_REFLECTION_SHADOW)
     return fail(JR_E_INVALID, "shadow_source is read only by phong_reflection_shadow");
-  if (!src->frame || !src->scenes || !src->workspace || !src->keys || !src->texels)
-    return fail(JR_E_INVALID, "shadow_source: frame, scenes, workspace, keys and texels "
-                              "are required");
+  if (!src->frame || !src->scenes || !src->workspace || !src->keys)
+    return fail(JR_E_INVALID, "shadow_source: frame, scenes, workspace and keys are required");
   const jr_frame* sf = src->frame;
   int32_t rc = check_frame(sf);
   if (rc) return rc;

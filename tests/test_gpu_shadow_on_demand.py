@@ -22,9 +22,10 @@ def _calls(n=24, seed=31, W=84, H=84, shadow=256):
     return batch.plan_batch(scs, W, H, sm)
 
 
-def _renderer(calls, on_demand, tile=(16, 16), shadow_tile=(16, 16), ppl=4, form=1):
+def _renderer(calls, on_demand, tile=(16, 16), shadow_tile=(16, 16), ppl=4, form=1,
+              keep=True):
     br = batch.BatchRenderer(calls, tile=tile, shadow_tile=shadow_tile, persistent=form == 0,
-                             shadow_on_demand=on_demand)
+                             shadow_on_demand=on_demand, keep_shadow_texels=keep)
     for p in br.passes.values():
         p.pixels_per_lane = ppl if p.tile[0] * p.tile[1] == 256 else 1
         p.tiles_per_group = max(1, form)
@@ -117,3 +118,20 @@ def test_on_demand_backward_equals_full_map_backward():
         assert torch.isfinite(a).all(), k
         scale = max(float(b.norm()), 1e-6)
         assert float((a - b).norm()) <= 1e-5 * scale, k
+
+
+def test_on_demand_without_kept_texels():
+    """A forward-only renderer (texels not kept) renders the same image and
+    writes no texel."""
+    calls = _calls(n=8, seed=36)
+    kept = _renderer(calls, True)
+    want = {k: v.clone() for k, v in kept.render().items()}
+    od = _renderer(calls, True, keep=False)
+    got = od.render()
+    torch.cuda.synchronize()
+    assert "shadow_texels" not in got
+    assert torch.equal(_bits(got["zbuffer"]), _bits(want["zbuffer"]))
+    assert torch.equal(_bits(got["canvas"]), _bits(want["canvas"]))
+    assert int((od.shadow_texels != 0).sum()) == 0
+    od.enable_backward()  # the backward needs them: kept from here on
+    assert "shadow_texels" in od.render()
