@@ -185,17 +185,19 @@ enum {
  * pass runs only its binning stages (jr_frame.part = JR_PART_BIN).  Each depth
  * found is also written to texels[scene][x][y] (unless texels is NULL), so
  * the map is right at every texel a pixel read: jr_backward, given
- * shadow_maps = texels, reads the same values.  The shadow pass must be a depth-only pass whose small-triangle keys
- * stay in its zbuffer: JR_SHADER_DEPTH, init_buffers = 1, apply_offset = 1, a
- * depth_offset other than -0, no prim_id / bary outputs; and its JR_PART_BIN
- * call must have run on `workspace` and `keys` (stream ordered before this
- * call).  All pointers but `frame` are device pointers. */
+ * shadow_maps = texels, reads the same values.  The shadow pass must be a
+ * depth-only pass in the narrow form (its small-triangle keys kept as 32-bit
+ * depth keys, tagged per binning, in its workspace): JR_SHADER_DEPTH,
+ * init_buffers = 1, apply_offset = 1, a depth_offset other than -0, no
+ * prim_id / bary outputs; and its JR_PART_BIN call must have run on
+ * `workspace` (stream ordered before this call).  All pointers but `frame`
+ * are device pointers. */
 typedef struct jr_shadow_source {
   const jr_frame* frame;    /* the shadow pass's frame (host) */
   const jr_scene* scenes;   /* its n_scenes scene records */
   const void* workspace;    /* its workspace */
   int64_t workspace_bytes;
-  const float* keys;        /* its zbuffer after JR_PART_BIN: per-texel 32-bit keys */
+  const float* keys;        /* not read (may be NULL): the keys stay in the workspace */
   float* texels;            /* [n_scenes][frame->width][frame->height], written;
                                NULL: not kept (a forward-only render) */
   int32_t n_tris;           /* its triangle count */
@@ -243,7 +245,15 @@ const char* jr_last_error(void);
 int64_t jr_sizeof(int32_t which);
 
 /* Bytes of workspace jr_forward needs for this frame (bin_capacity included).
- * Replaces nothing in the reference: JAX sizes its temporaries itself. */
+ * Replaces nothing in the reference: JAX sizes its temporaries itself.  A
+ * depth pass in the narrow form (JR_SHADER_DEPTH, init_buffers, apply_offset,
+ * an offset other than -0, no prim_id / bary) keeps its small-triangle keys
+ * in the workspace from call to call, each binning's tagged apart from the
+ * older ones, so that it never clears them: zero a newly allocated workspace
+ * (memory that held a workspace of the same size before could pass for one
+ * whose keys are in order), keep one workspace per stream and do not write
+ * to it between calls.  A workspace used by another kind of pass in between,
+ * or sized for another pixel count, is detected and its keys are refilled. */
 int64_t jr_workspace_size(const jr_frame* frame, int32_t n_tris);
 
 /* Render n_scenes scenes in one call: vertex transform, triangle setup,
@@ -268,7 +278,9 @@ int32_t jr_forward(const jr_frame* frame, const jr_inputs* inputs,
  * is never recorded.  Every call on a GPU shares that GPU's scratch: calls
  * that overlap in time on one GPU (two streams, two threads) race and must be
  * serialised by the caller.  Each call clears what it reads before writing
- * it, so no state carries over from the previous call. */
+ * it, except the narrow form's tagged small-triangle keys (see
+ * jr_workspace_size), whose older tags lose to the call's own: no result
+ * depends on an earlier call. */
 #define JR_SCRATCH_BYTES (64LL << 20)
 int32_t jr_forward_noalloc(const jr_frame* frame, const jr_inputs* inputs,
                            const jr_scene* scenes, jr_outputs* outputs,
@@ -296,8 +308,9 @@ int32_t jr_stage_times(float* ms, int32_t max_calls);
 /* Reverse-mode pass of jr_forward with visibility held fixed (the primitive
  * chosen per pixel, comparisons, floor/round lookups carry no gradient, as
  * under jax.grad).  Takes the upstream gradients of the canvas (and
- * optionally the zbuffer) and the prim_id buffer jr_forward wrote; for every
- * written pixel it recomputes the chosen primitive's edge values,
+ * optionally the zbuffer) and the prim_id buffer jr_forward wrote (entries
+ * outside [0, n_tris) count as pixels no triangle shows); for every written
+ * pixel it recomputes the chosen primitive's edge values,
  * barycentrics and fragment attributes exactly as the forward did and
  * accumulates (+=) gradients into `grads`.  Replaces jax.grad through
  * renderer/pipeline.py:163-328 (edge functions, depth, interpolation), the

@@ -151,10 +151,10 @@ def workspace_for(stream: torch.cuda.Stream, nbytes: int, dev: torch.device,
     with _ws_lock:
         ws = _workspaces.setdefault(key, _Workspace())
         if ws.buf is None or ws.buf.numel() < nbytes:
-            # the backward's is zeroed once (its header then names the state
-            # it holds); recycled memory must not pass for a cleared workspace
-            alloc = torch.zeros if kind == "backward" else torch.empty
-            ws.buf = alloc(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
+            # zeroed once: the backward's header and the forward's key state
+            # then name the state they hold; recycled memory must not pass
+            # for a cleared workspace
+            ws.buf = torch.zeros(max(nbytes, 1 << 20), dtype=torch.uint8, device=dev)
         return ws.buf
 
 
@@ -287,14 +287,15 @@ def forward(
 @dataclass
 class ShadowSource:
     """A shadow pass (depth shader, offset, no id outputs) whose binning stages
-    ran on `workspace` with `keys` as its zbuffer, for a phong_reflection_shadow
-    pass that computes the texel depths it reads (jr_shadow_source).  `texels`
-    [n_scenes, width, height] receives every depth so computed (None: they are
-    not kept)."""
+    ran on `workspace`, for a phong_reflection_shadow pass that computes the
+    texel depths it reads (jr_shadow_source): the pass's small-triangle keys
+    and tile lists stay in its workspace.  `texels` [n_scenes, width, height]
+    receives every depth so computed (None: they are not kept).  `keys` is
+    not read (the shadow pass's zbuffer, which its binning leaves as it was)."""
     frame: _hip.JrFrame
     scenes: torch.Tensor
     workspace: torch.Tensor
-    keys: torch.Tensor
+    keys: Optional[torch.Tensor]
     texels: Optional[torch.Tensor]
     n_tris: int
 
@@ -307,9 +308,8 @@ class ShadowSource:
         s.keys = _ptr(self.keys, "shadow keys", torch.float32, dev)
         s.texels = _ptr(self.texels, "shadow texels", torch.float32, dev)
         n, w, h = self.frame.n_scenes, self.frame.width, self.frame.height
-        if tuple(self.keys.shape) != (n, w, h) or (
-                self.texels is not None and tuple(self.texels.shape) != (n, w, h)):
-            raise ValueError(f"shadow keys and texels must be [{n}, {w}, {h}]")
+        if self.texels is not None and tuple(self.texels.shape) != (n, w, h):
+            raise ValueError(f"shadow texels must be [{n}, {w}, {h}]")
         s.n_tris = int(self.n_tris)
         return s, (self.frame,)
 

@@ -201,7 +201,8 @@ class _Pass:
         self.capacity = int(cap)
         need = _raster.workspace_size(width=c.width, height=c.height, n_scenes=c.n_scenes,
                                       n_tris=c.n_tris, tile=self.tile, capacity=self.capacity)
-        self.workspace = torch.empty(need, dtype=torch.uint8, device=self.zbuffer.device)
+        # zeroed once (the depth pass's key state then describes what it holds)
+        self.workspace = torch.zeros(need, dtype=torch.uint8, device=self.zbuffer.device)
 
     def frame(self, part: int = _hip.PART_ALL) -> _hip.JrFrame:
         """The jr_frame of this pass's launches."""

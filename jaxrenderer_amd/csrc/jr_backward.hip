@@ -701,7 +701,8 @@ __global__ __launch_bounds__(64) void k_bwd_pixels(BwdArgs a) {
     *i_out = i;
     *j_out = j;
     if (i >= a.W || j >= a.H) return -1;
-    return a.prim_id[(int64_t)s * a.W * a.H + (int64_t)i * a.H + j];
+    const int32_t t = a.prim_id[(int64_t)s * a.W * a.H + (int64_t)i * a.H + j];
+    return t < a.in.n_tris ? t : -1;
   };
 
   // the tile's distinct primitives; the texel and specular-map sum tables

@@ -234,12 +234,53 @@ JR_DEV bool has_nan_column(const float4& r) {
   return (r.x != r.x) || (r.y != r.y) || (r.z != r.z);
 }
 
-// Clears the tile counters (n32 int32), sets the per-pixel keys (n64 uint64,
-// or nk32 uint32 in the narrow form) to "no candidate" and fills every scene's
-// sample table: the NDC
-// coordinates pix_x(i) for i < W followed by pix_y(j) for j < H.
+// Small-triangle keys of the narrow form.  They live in the workspace's key
+// buffer as 64-bit words (tag << 32) | depth key, tag = ~epoch of the binning
+// that wrote them, and stay there from call to call: each binning of the
+// workspace takes the next epoch, whose smaller tag wins every atomic minimum
+// against an older word, and a word with another tag reads as "no
+// candidate".  So the buffer is not cleared per call, only when the key
+// state does not describe it: a new or foreign workspace, another pixel
+// count or buffer offset, any other binning on the workspace in between (the
+// state sits at the workspace's start, where every layout keeps it, and the
+// wide form's k_clear marks it stale), or the epoch running out.  The fill
+// value ~0 has tag 0xFFFFFFFF, which no epoch >= 1 produces.
+constexpr int kKsMagic = 0, kKsPixLo = 1, kKsPixHi = 2, kKsOffLo = 3, kKsOffHi = 4,
+              kKsEpoch = 5, kKsFill = 6;
+constexpr uint32_t kKeyMagic = 0x6b657973u;
+
+JR_DEV uint64_t tagged_key(uint32_t key, uint32_t tag) {
+  return ((uint64_t)tag << 32) | key;
+}
+JR_DEV uint32_t untag_key(uint64_t k, uint32_t tag) {
+  return (uint32_t)(k >> 32) == tag ? (uint32_t)k : ~0u;
+}
+
+// Before a narrow binning: the next epoch, or a full fill of the key buffer
+// (by k_clear) with epoch 1 when the state does not describe the buffer.
+__global__ void k_keys_prepare(uint32_t* __restrict__ ks, int64_t n_pix, int64_t keys_at) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t lo = (uint32_t)n_pix, hi = (uint32_t)((uint64_t)n_pix >> 32);
+  const uint32_t olo = (uint32_t)keys_at, ohi = (uint32_t)((uint64_t)keys_at >> 32);
+  const uint32_t e = ks[kKsEpoch];
+  const bool ok = ks[kKsMagic] == kKeyMagic && ks[kKsPixLo] == lo && ks[kKsPixHi] == hi &&
+                  ks[kKsOffLo] == olo && ks[kKsOffHi] == ohi && e >= 1u && e < 0xFFFFFFFEu;
+  ks[kKsFill] = ok ? 0u : 1u;
+  ks[kKsEpoch] = ok ? e + 1u : 1u;
+  ks[kKsMagic] = kKeyMagic;
+  ks[kKsPixLo] = lo;
+  ks[kKsPixHi] = hi;
+  ks[kKsOffLo] = olo;
+  ks[kKsOffHi] = ohi;
+}
+
+// Clears the tile counters (n32 int32), sets the n64 per-pixel keys to "no
+// candidate" (the narrow form: only when k_keys_prepare asked for a fill; the
+// wide form: always, and the key state no longer describes the buffer) and
+// fills every scene's sample table: the NDC coordinates pix_x(i) for i < W
+// followed by pix_y(j) for j < H.
 __global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __restrict__ p64,
-                        int64_t n64, uint32_t* __restrict__ k32, int64_t nk32,
+                        int64_t n64, uint32_t* __restrict__ ks,
                         const jr_scene* __restrict__ scenes, Frame f,
                         float* __restrict__ samples, int32_t* __restrict__ wave_scene,
                         int32_t n_tris) {
@@ -259,8 +300,12 @@ __global__ void k_clear(int32_t* __restrict__ p32, int64_t n32, uint64_t* __rest
     }
   }
   for (int64_t i = first; i < n32; i += stride) p32[i] = 0;
+  if (f.narrow) {
+    if (ks[kKsFill] == 0u) n64 = 0;
+  } else if (first == 0) {
+    ks[kKsMagic] = 0u;
+  }
   for (int64_t i = first; i < n64; i += stride) p64[i] = ~0ull;
-  for (int64_t i = first; i < nk32; i += stride) k32[i] = ~0u;
   const int32_t per = f.W + f.H;
   for (int64_t i = first; i < (int64_t)f.n_scenes * per; i += stride) {
     const int32_t s = (int32_t)(i / per), q = (int32_t)(i - (int64_t)s * per);
@@ -391,7 +436,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     float4* __restrict__ rec, Rect* __restrict__ rects,
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
     int2* __restrict__ wide, int32_t* __restrict__ wide_count,
-    const float* __restrict__ samples, uint32_t* __restrict__ zkeys,
+    const float* __restrict__ samples, const uint32_t* __restrict__ ks,
     uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene) {
   __shared__ SetupStage stage;
   __shared__ int32_t wave_live[kSetupThreads / 64];
@@ -595,6 +640,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     vz1 = scenes[s_lo].viewport[11];
     __syncthreads();
   }
+  const uint32_t tag = f.narrow ? ~ks[kKsEpoch] : 0u;  // this binning's key tag
   // the tile raster's exact per-pixel arithmetic for every (triangle, pixel)
   auto small_pairs = [&](auto from_lds) {
     constexpr bool LDS = decltype(from_lds)::value;
@@ -631,7 +677,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
         const uint32_t key = depth_key(win_depth(q0, q1, q2, e0, e1, e2, v0, v1));
         const int64_t pix = ((int64_t)box.w * f.W + i) * f.H + j;
         if (f.narrow)
-          atomicMin(zkeys + pix, key);
+          atomicMin((unsigned long long*)(keys + pix), (unsigned long long)tagged_key(key, tag));
         else
           atomicMin((unsigned long long*)(keys + pix),
                     (unsigned long long)pack_key(key, sm.u.t.tri[o]));
@@ -867,7 +913,8 @@ struct ShadowLookup {
   const Rect* rects;
   const uint32_t* masks;
   const float* samples;
-  const uint32_t* keys;
+  const uint64_t* keys;  // its tagged small-triangle keys
+  const uint32_t* ks;    // its key state (the current epoch)
   const jr_scene* scenes;
   float* texels;
   Frame f;
@@ -898,7 +945,7 @@ JR_DEV float shadow_depth_on_demand(const ShadowLookup& L, int32_t s, float x, f
   const float vp22 = sc.viewport[10], vp23 = sc.viewport[11];
   uint32_t best_key = 0xFFFFFFFFu;
   int32_t best_id = 0x7FFFFFFF;
-  const uint32_t k0 = L.keys[pix];
+  const uint32_t k0 = untag_key(L.keys[pix], ~L.ks[kKsEpoch]);
   if (k0 != ~0u) {
     best_key = k0;
     best_id = -1;  // small winner: its id is not kept, no equal key displaces it
@@ -1009,6 +1056,7 @@ struct RasterArgs {
   const int32_t* offsets;
   const int32_t* bins;
   const uint64_t* keys;
+  const uint32_t* ks;    // key state (the narrow form's epoch)
   const float* samples;  // per scene: pix_x(i) for i < W, then pix_y(j)
   const Rect* rects;     // per-triangle tile ranges and binning words (k_setup)
   const uint32_t* masks;
@@ -1256,6 +1304,7 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
   // with NT a multiple of TH all of a thread's pixels share one j (and y)
   constexpr bool kOneRow = (NT % TH) == 0;
   const float* sx = a.samples + (int64_t)s * (f.W + f.H);
+  const uint32_t ktag = f.narrow ? ~a.ks[kKsEpoch] : 0u;
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     const int p = tid + q * NT;
@@ -1271,7 +1320,7 @@ JR_DEV void raster_tile(const RasterArgs& a, int64_t gt, float4* lds_rec,
     if (inside[q]) {
       const int64_t pix = (int64_t)s * f.W * f.H + (int64_t)pi[q] * f.H + pj[q];
       if (f.narrow) {
-        const uint32_t k0 = reinterpret_cast<const uint32_t*>(a.out.zbuffer)[pix];
+        const uint32_t k0 = untag_key(a.keys[pix], ktag);
         if (k0 != ~0u) {
           best_key[q] = k0;
           best_id[q] = -1;
@@ -1489,8 +1538,8 @@ constexpr int64_t kAlign = 256;
 int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
-  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, wide, samples, masks,
-      wave_scene, wbits, bins, total;
+  int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, kstate, wide, samples,
+      masks, wave_scene, wbits, bins, total;
   int64_t n_tiles, nblk, n_pix;
 };
 
@@ -1539,6 +1588,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.n_tiles = tiles_x * tiles_y * (int64_t)fr->n_scenes;
   L.nblk = (L.n_tiles + kScanBlock - 1) / kScanBlock;
   int64_t o = 0;
+  L.kstate = o;     o += align_up(64);   // key state (k_keys_prepare): first in every layout
   L.rec = o;        o += align_up(64 * (int64_t)std::max(n_tris, 1));
   L.rects = o;      o += align_up(16 * (int64_t)std::max(n_tris, 1));
   L.counts = o;     o += align_up(4 * L.n_tiles);
@@ -1667,8 +1717,8 @@ int32_t check_shadow_source(const jr_frame* fr, const jr_inputs* in) {
   const jr_shadow_source* src = in->shadow_source;
   if (fr->shader != JR_SHADER_PHONG_REFLECTION_SHADOW)
     return fail(JR_E_INVALID, "shadow_source is read only by phong_reflection_shadow");
-  if (!src->frame || !src->scenes || !src->workspace || !src->keys)
-    return fail(JR_E_INVALID, "shadow_source: frame, scenes, workspace and keys are required");
+  if (!src->frame || !src->scenes || !src->workspace)
+    return fail(JR_E_INVALID, "shadow_source: frame, scenes and workspace are required");
   const jr_frame* sf = src->frame;
   int32_t rc = check_frame(sf);
   if (rc) return rc;
@@ -1765,6 +1815,7 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   int32_t* block_sums = reinterpret_cast<int32_t*>(ws + L.block_sums);
   int32_t* bins = reinterpret_cast<int32_t*>(ws + L.bins);
   uint64_t* keys = reinterpret_cast<uint64_t*>(ws + L.keys);
+  uint32_t* kstate = reinterpret_cast<uint32_t*>(ws + L.kstate);
   int2* wide = reinterpret_cast<int2*>(ws + L.wide);
   float* samples = reinterpret_cast<float*>(ws + L.samples);
   uint32_t* masks = reinterpret_cast<uint32_t*>(ws + L.masks);
@@ -1779,20 +1830,23 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   if (fr->part != JR_PART_RASTER) {
   // counts, cursor and misc are contiguous: clear them, and the key buffer,
   // with one fill kernel (a kernel rather than hipMemsetAsync so the sequence
-  // can be graph-captured)
+  // can be graph-captured); the narrow form's tagged keys are filled only
+  // when their state asks for it
   const int64_t nz = (L.offsets - L.counts) / 4;
-  const int64_t nclr = std::max(std::max(nz, L.n_pix),
+  // (the narrow form fills its keys only now and then: a smaller grid)
+  const int64_t nclr = std::max(std::max(nz, f.narrow ? std::min<int64_t>(L.n_pix, 1 << 20)
+                                                      : L.n_pix),
                                 (int64_t)(f.W + f.H) * f.n_scenes);
-  uint32_t* zkeys = reinterpret_cast<uint32_t*>(out->zbuffer);
+  if (f.narrow)
+    hipLaunchKernelGGL(k_keys_prepare, dim3(1), dim3(64), 0, st, kstate, L.n_pix, L.keys);
   hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<int64_t>((nclr + 255) / 256, 8192)),
-                     dim3(256), 0, st, counts, nz, keys, f.narrow ? (int64_t)0 : L.n_pix,
-                     zkeys, f.narrow ? L.n_pix : (int64_t)0, scenes, f, samples, wave_scene,
-                     in->n_tris);
+                     dim3(256), 0, st, counts, nz, keys, L.n_pix, kstate, scenes, f, samples,
+                     wave_scene, in->n_tris);
   timer.mark(1);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
                        dim3(kSetupThreads), 0, st, *in,
-                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples, zkeys,
+                       scenes, f, rec, rects, counts, keys, wide, wide_count, samples, kstate,
                        masks, wave_scene);
   timer.mark(2);
   if (n_tris > 0)
@@ -1833,6 +1887,7 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   a.rects = rects;
   a.masks = masks;
   a.keys = keys;
+  a.ks = kstate;
   a.samples = samples;
   a.work_counter = misc + 1;
   a.chunk = 1;
@@ -1859,7 +1914,8 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
     a.sl.rects = reinterpret_cast<const Rect*>(sw + SL.rects);
     a.sl.masks = reinterpret_cast<const uint32_t*>(sw + SL.masks);
     a.sl.samples = reinterpret_cast<const float*>(sw + SL.samples);
-    a.sl.keys = reinterpret_cast<const uint32_t*>(src.keys);
+    a.sl.keys = reinterpret_cast<const uint64_t*>(sw + SL.keys);
+    a.sl.ks = reinterpret_cast<const uint32_t*>(sw + SL.kstate);
     a.sl.scenes = src.scenes;
     a.sl.texels = src.texels;
     a.sl.f = dev_frame(src.frame, true);

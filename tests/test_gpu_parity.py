@@ -125,9 +125,9 @@ def test_many_scenes_xcd_mapping_and_tiles_per_group(tiles_per_group, ppl):
 @pytest.mark.parametrize("tile,ppl", [((8, 8), 1), ((16, 16), 4)], ids=str)
 def test_depth_only_offset_pass(seed, tile, ppl):
     """A depth-shader pass with an offset and no id outputs (a shadow map):
-    small triangles keep 32-bit depth keys in the output buffer itself.  The
-    depth must still match the oracle bit for bit, on tiny and pixel-aligned
-    triangles and on config-3 shadow maps."""
+    small triangles keep 32-bit depth keys, tagged per binning, in the
+    workspace.  The depth must still match the oracle bit for bit, on tiny
+    and pixel-aligned triangles and on config-3 shadow maps."""
     from jaxrenderer_amd import batch, workloads
 
     def calls():
@@ -147,6 +147,34 @@ def test_depth_only_offset_pass(seed, tile, ppl):
     got = _plan.run_plan(shadow, tile=tile, pixels_per_lane=ppl)
     torch.cuda.synchronize()
     assert_parity(got["shadow"], ref["shadow"], label=f"c3shadow{seed}")
+
+
+def test_depth_pass_keys_carry_over_calls():
+    """The narrow form never clears its small-triangle keys: they stay in the
+    workspace, each binning's tagged apart from the older ones.  On one
+    workspace, shadow passes of scenes A, then B (same map size, other
+    triangles), a whole config-3 frame of A (its main pass bins in the wide
+    form on the same workspace), A, B and A again each match the oracle bit
+    for bit."""
+    from jaxrenderer_amd import batch, workloads
+
+    def calls(seed, full):
+        scs, sm = workloads.config3(n_scenes=3, seed=seed)
+        out = batch.plan_batch(scs, 84, 84, sm)
+        return out if full else [c for c in out if c.tag == "shadow"]
+
+    refs = {}
+    for step, (seed, full) in enumerate(((41, False), (42, False), (41, True), (41, False),
+                                         (42, False), (41, False))):
+        label = f"step{step}/seed{seed}{'/frame' if full else ''}"
+        print(label, flush=True)
+        if (seed, full) not in refs:
+            refs[(seed, full)] = run_oracle_plan(calls(seed, full), prim_id=False, bary=False)
+        ref = refs[(seed, full)]
+        got = _plan.run_plan(calls(seed, full), tile=(16, 16), pixels_per_lane=4)
+        torch.cuda.synchronize()
+        for tag in ref:
+            assert_parity(got[tag], ref[tag], label=f"{label}/{tag}")
 
 
 @pytest.mark.parametrize("shader", [DepthShader, PhongReflectionShadowTextureShader],
