@@ -209,6 +209,25 @@ JR_DEV float max_nan(float a, float b) {
   return a > b ? a : b;
 }
 
+// jnp.power(x, y) of the specular term, x = max(cos, 0) in [+0, 1] or NaN.
+// For y > 0 and a normal x it is 2^(y log2 x) from the hardware base-2
+// logarithm and exponential (each within about 1 ulp) whenever
+// t = y log2 x <= 0.5: the result is then at most 1.42 and its relative
+// error about (1 + |t|) 1.2e-7 (the product's rounding and the logarithm's
+// error are scaled by |t| ln 2), an absolute error below 3e-7, where colours
+// are held to 1e-5.  x = +0 with y > 0 gives +0, as powf does.  Everything
+// else (y <= 0 or NaN, subnormal or NaN x, larger results) is powf.
+JR_DEV float pow_unit(float x, float y) {
+  if (y > 0.0f) {
+    if (x == 0.0f) return 0.0f;
+    if (x >= 1.17549435e-38f) {
+      const float t = y * __builtin_amdgcn_logf(x);
+      if (t <= 0.5f) return __builtin_amdgcn_exp2f(t);
+    }
+  }
+  return powf(x, y);
+}
+
 // jnp.modf fractional part: x - (x >= 0 ? floor(x) : ceil(x)).
 JR_DEV float modf_frac(float x) {
   const float whole = (x >= 0.0f) ? floorf(x) : ceilf(x);

@@ -174,7 +174,7 @@ JR_DEV void reflection_terms(const jr_inputs& in, const jr_scene& sc,
   r[2] = two_d * n[2] - l[2];
   float rn[3];
   normalise3(r, rn);
-  *specular = powf(max_nan(rn[2], 0.0f), spec_exp);
+  *specular = pow_unit(max_nan(rn[2], 0.0f), spec_exp);
 }
 
 template <int SHADER>
