@@ -79,10 +79,16 @@ inline void magic31(uint32_t d, uint32_t* m, int32_t* sh) {
   *sh = 31 + l;
 }
 
-// q / h for 0 <= q < 16 and 1 <= h <= 16: (q * ceil(256 / h)) >> 8 is exact,
-// since the excess over q / h is below q / 256 < 1 / h.
+// q / h for 0 <= q < 2^22 and 1 <= h < 2^22, without an integer division:
+// q and h are exact floats, and the float quotient by the hardware
+// reciprocal (within 1 ulp, the product within half an ulp more) is within
+// 2e-7 q / h < 1 of the exact one, so its truncation is the quotient or one
+// off, and one correction step either way makes it exact.
 JR_DEV int32_t div_small(int32_t q, int32_t h) {
-  return (q * ((256 + h - 1) / h)) >> 8;
+  int32_t d = (int32_t)((float)q * __builtin_amdgcn_rcpf((float)h));
+  if (d * h > q) --d;
+  if ((d + 1) * h <= q) ++d;
+  return d;
 }
 
 // The scene of triangle t (the last scene whose tri_begin <= t; scene ranges

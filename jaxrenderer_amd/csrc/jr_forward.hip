@@ -650,11 +650,7 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
       if (item >= total) continue;
       const int4 box = sm.box[o];
       const int32_t q = item - sm.off[o];
-      // q / h for q < 64, h <= 64: a float estimate (the hardware reciprocal,
-      // within a few ulps), corrected to the exact quotient
-      int32_t di = (int32_t)((float)q * __builtin_amdgcn_rcpf((float)box.z));
-      if (di * box.z > q) --di;
-      if ((di + 1) * box.z <= q) ++di;
+      const int32_t di = div_small(q, box.z);
       const int32_t i = box.x + di, j = box.y + (q - di * box.z);
       const float4 q0 = sm.r[0][o], q1 = sm.r[1][o], q2 = sm.r[2][o];
       float x, y;
@@ -733,7 +729,8 @@ __global__ __launch_bounds__(256) void k_wide(
       const int32_t q = q0 + lane;
       bool pass = false;
       if (q < n_t) {
-        const int32_t tx = r.tx0 + q / h, ty = r.ty0 + q % h;
+        const int32_t dq = div_small(q, h);
+        const int32_t tx = r.tx0 + dq, ty = r.ty0 + (q - dq * h);
         if (BIN && base >= 0)
           pass = ((wbits[base + (q >> 5)] >> (q & 31)) & 1u) != 0u;
         else
@@ -1622,6 +1619,8 @@ int32_t check_frame(const jr_frame* fr) {
   const int64_t n_tiles = (int64_t)((fr->width + fr->tile_w - 1) / fr->tile_w) *
                           ((fr->height + fr->tile_h - 1) / fr->tile_h) * fr->n_scenes;
   if (n_tiles > 0x7FFFFFFFLL) return fail(JR_E_INVALID, "more than 2^31 - 1 tiles");
+  if (n_tiles / fr->n_scenes >= (1LL << 22))
+    return fail(JR_E_INVALID, "2^22 or more tiles per scene");
   if (fr->bin_capacity < 0 || fr->bin_capacity > 0x7FFFFFFFLL)
     return fail(JR_E_INVALID, "bin_capacity out of range");
   if (fr->part != JR_PART_ALL && fr->part != JR_PART_BIN && fr->part != JR_PART_RASTER)
