@@ -597,11 +597,15 @@ struct Pending {
   }
 };
 
+constexpr int32_t kStageProbes = 16;
+
 template <int CAP, int NC, int KMAX>
 JR_DEV void stage_add(int32_t* tab, float (*val)[NC], float* __restrict__ d, int64_t at,
                       int32_t n, const float* v, Pending<NC, KMAX>& pend) {
   if (!d || at < 0) return;
-  const int32_t e = at < 0x7FFFFFFFLL ? table_insert(tab, CAP, (int32_t)at) : -1;
+  // a texel table fills up where nearly every pixel reads its own texel: a
+  // miss then costs kStageProbes LDS operations, not CAP
+  const int32_t e = at < 0x7FFFFFFFLL ? table_insert(tab, CAP, (int32_t)at, kStageProbes) : -1;
   if (e < 0) {
     if (!pend.push(at, n, v))
       for (int c = 0; c < n; ++c)

@@ -114,11 +114,15 @@ JR_DEV int32_t scene_of_wave(const jr_scene* __restrict__ scenes, int32_t n,
   return s;
 }
 
-// Open-addressing set of triangle ids in LDS (-1: free): the entry holding w,
-// or -1 when the table is full.
-JR_DEV int32_t table_insert(int32_t* tab, int32_t cap, int32_t w) {
+// Open-addressing set of non-negative ids in LDS (-1: free), linear probing
+// from w mod cap over at most `probes` entries (all of them by default): the
+// entry holding w, or -1 when those entries are all taken by other ids.  A
+// table's insertions all use one window length, so an id present is always
+// found; a short window bounds what a miss costs once the table is full.
+JR_DEV int32_t table_insert(int32_t* tab, int32_t cap, int32_t w, int32_t probes = 0) {
   int32_t e = (int32_t)((uint32_t)w % (uint32_t)cap);
-  for (int32_t n = 0; n < cap; ++n) {
+  const int32_t n_probe = (probes > 0 && probes < cap) ? probes : cap;
+  for (int32_t n = 0; n < n_probe; ++n) {
     const int32_t cur = atomicCAS(tab + e, -1, w);
     if (cur == -1 || cur == w) return e;
     e = (e + 1 == cap) ? 0 : e + 1;
