@@ -44,6 +44,30 @@ def test_soup_parity(shader, seed):
     _check(lambda: scenes.soup_call(shader, seed), f"{shader.__name__}/{seed}")
 
 
+@pytest.mark.parametrize("shader", ["PhongReflectionTextureShader",
+                                    "PhongReflectionShadowTextureShader"])
+def test_specular_exponents_fast_and_exact_power(shader):
+    """The specular power max(cos, 0) ^ s takes the hardware log2 / exp2 path
+    where the result is at most 1.42 and powf elsewhere: exponents 0 (powf:
+    exactly 1), 1e-3, 0.5, 1, 4, 64 and 200 on one soup keep the colours
+    within 1e-5 of the oracle's powf."""
+    from jaxrenderer_amd import shaders
+
+    cls = getattr(shaders, shader)
+    vals = np.array([0.0, 1e-3, 0.5, 1.0, 4.0, 64.0, 200.0], np.float32)
+
+    def calls():
+        out = scenes.soup_call(cls, 5, 64, 48, 150)
+        for c in out:
+            sm = c.inputs.get("specular_maps")
+            if sm is not None:
+                c.inputs["specular_maps"] = vals[np.arange(sm.size) % vals.size].reshape(sm.shape)
+        return out
+
+    got, _ = _check(calls, f"spec/{shader}")
+    assert (got["main"]["prim_id"].cpu().numpy() >= 0).sum() > 500
+
+
 @pytest.mark.parametrize("tile", TILES, ids=str)
 @pytest.mark.parametrize("persistent", [False, True])
 def test_tiles_and_persistent(tile, persistent):
