@@ -56,9 +56,11 @@ namespace jr {
 // bound on their rounding (u = 2^-24 per operation; only systems far from
 // singular, |det| > 1e-3 of the product of the rows' 1-norms, and no
 // coefficient of magnitude in (0, 1e-12), so no intermediate is subnormal).
-// Otherwise (R unbounded, empty or ill-conditioned: a few triangles in a
-// thousand) the range is the whole viewport, and the exact per-tile test of
-// the binning keeps only the tiles that may hold a covered pixel.
+// Otherwise (R unbounded, empty or ill-conditioned, or coefficients of 1e18
+// and more: about one triangle in a hundred) the range is the box of the
+// viewport rectangles that the exact rectangle test cannot reject
+// (fallback_range), and the exact per-tile test of the binning keeps only the
+// tiles that may hold a covered pixel.
 JR_DEV bool shifted_triangle_box_f(const float a[3], const float b[3], const float c[3],
                                    float* xmin, float* xmax, float* ymin, float* ymax) {
   constexpr float g = 8.0f / 16777216.0f;
@@ -122,13 +124,50 @@ JR_DEV Rect pixel_box(double xmin, double xmax, double ymin, double ymax, const 
   return r;
 }
 
+JR_DEV bool rect_may_cover(const float4& r0, const float4& r1, const float4& r2,
+                           bool sane, const float* vp, const float* __restrict__ sx,
+                           const Frame& f, int32_t i0, int32_t i1, int32_t j0, int32_t j1);
+
+// The viewport split into kFallbackSplit x kFallbackSplit rectangles, each
+// kept only where the exact rectangle test finds that a pixel of it may pass:
+// the range of a triangle whose region R has no usable bound (about one
+// triangle in a hundred in a Brax-like frame: edge-on or crossing the eye
+// plane; half of them cover no pixel at all).
+constexpr int32_t kFallbackSplit = 1;
+
+JR_DEV Rect fallback_range(const float4& r0, const float4& r1, const float4& r2, bool sane,
+                           const float* vp, const float* __restrict__ sx, const Frame& f) {
+  Rect r = {1, 0, 1, 0};
+#pragma unroll
+  for (int32_t qi = 0; qi < kFallbackSplit; ++qi) {
+    const int32_t i0 = (f.W * qi) / kFallbackSplit, i1 = (f.W * (qi + 1)) / kFallbackSplit - 1;
+#pragma unroll
+    for (int32_t qj = 0; qj < kFallbackSplit; ++qj) {
+      const int32_t j0 = (f.H * qj) / kFallbackSplit, j1 = (f.H * (qj + 1)) / kFallbackSplit - 1;
+      if (i0 > i1 || j0 > j1 || !rect_may_cover(r0, r1, r2, sane, vp, sx, f, i0, i1, j0, j1))
+        continue;
+      if (r.tx0 > r.tx1) {
+        r = {i0, i1, j0, j1};
+      } else {
+        r.tx0 = min(r.tx0, i0);
+        r.tx1 = max(r.tx1, i1);
+        r.ty0 = min(r.ty0, j0);
+        r.ty1 = max(r.ty1, j1);
+      }
+    }
+  }
+  return r;
+}
+
+// sx: the scene's sample table
 JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
-                        bool sane, const float* vp, const Frame& f) {
-  Rect full = {0, f.W - 1, 0, f.H - 1};
-  if (!sane || !(vp[0] > 0.0f) || !(vp[5] > 0.0f)) return full;
+                        bool sane, const float* vp, const float* __restrict__ sx,
+                        const Frame& f) {
+  if (!sane || !(vp[0] > 0.0f) || !(vp[5] > 0.0f))
+    return fallback_range(r0, r1, r2, sane, vp, sx, f);
   const double X0 = pix_x(0, vp), X1 = pix_x(f.W - 1, vp);
   const double Y0 = pix_y(0, vp), Y1 = pix_y(f.H - 1, vp);
-  if (!(X0 <= X1) || !(Y0 <= Y1)) return full;
+  if (!(X0 <= X1) || !(Y0 <= Y1)) return fallback_range(r0, r1, r2, sane, vp, sx, f);
   const double XM = fmax(fabs(X0), fabs(X1)), YM = fmax(fabs(Y0), fabs(Y1));
   const float4 rs[3] = {r0, r1, r2};
   Rect r = {1, 0, 1, 0};
@@ -153,10 +192,9 @@ JR_DEV Rect pixel_range(const float4& r0, const float4& r1, const float4& r2,
     if (!(xmin <= xmax) || !(ymin <= ymax)) return r;
     return pixel_box(xmin, xmax, ymin, ymax, vp, f);
   }
-  // R unbounded, empty or ill-conditioned (a few triangles in a thousand):
-  // the whole viewport; the exact per-tile test then keeps only the tiles
-  // that may hold a covered pixel
-  return full;
+  // R unbounded, empty or ill-conditioned: the fallback range; the exact
+  // per-tile test then keeps only the tiles that may hold a covered pixel
+  return fallback_range(r0, r1, r2, sane, vp, sx, f);
 }
 
 JR_DEV Rect tiles_of(const Rect& p, const Frame& f) {
@@ -176,15 +214,16 @@ JR_DEV uint64_t pack_key(uint32_t key, int32_t id) {
   return ((uint64_t)key << 32) | (uint32_t)id;
 }
 
-// Exact tile rejection.  With finite coefficients every computed edge value
-// is monotone in i and in j (IEEE rounding is monotone), so its maximum over a
-// tile sits at a corner pixel.  With an infinite coefficient a (no NaN), x*a
+// Exact rejection of a rectangle of pixels (a tile, or a whole viewport).
+// With finite coefficients every computed edge value is monotone in i and in
+// j (IEEE rounding is monotone), so its maximum over a rectangle sits at a
+// corner pixel.  With an infinite coefficient a (no NaN), x*a
 // only depends on the sign class of x (-inf, NaN at x == 0, +inf) and the
 // finite terms beside it no longer matter, so one pixel per sign class
 // decides.  Testing the pixels {i0, i1, i with x == 0} x {j0, j1, j with
 // y == 0} therefore finds a passing value whenever any pixel of the tile has
-// one; a tile where some edge fails at all of them holds no pixel that passes
-// the reference's inside test.
+// one; a rectangle where some edge fails at all of them holds no pixel that
+// passes the reference's inside test.
 JR_DEV int32_t zero_pixel(float v3, int32_t lo, int32_t hi) {
   // the pixel whose NDC coordinate is exactly 0, if it lies in [lo, hi]
   const float r = floorf(v3);
@@ -192,12 +231,10 @@ JR_DEV int32_t zero_pixel(float v3, int32_t lo, int32_t hi) {
   return (int32_t)r;
 }
 
-JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
+JR_DEV bool rect_may_cover(const float4& r0, const float4& r1, const float4& r2,
                            bool sane, const float* vp, const float* __restrict__ sx,
-                           const Frame& f, int32_t tx, int32_t ty) {
+                           const Frame& f, int32_t i0, int32_t i1, int32_t j0, int32_t j1) {
   // sx: the scene's sample table (pix_x for i < W, then pix_y)
-  const int32_t i0 = tx * f.TW, i1 = min(i0 + f.TW, f.W) - 1;
-  const int32_t j0 = ty * f.TH, j1 = min(j0 + f.TH, f.H) - 1;
   float xs[3] = {sx[i0], sx[i1], 0.0f};
   float ys[3] = {sx[f.W + j0], sx[f.W + j1], 0.0f};
   int nx = 2, ny = 2;
@@ -216,6 +253,15 @@ JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
     if (!pass) return false;
   }
   return true;
+}
+
+// rect_may_cover for the pixels of tile (tx, ty)
+JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
+                           bool sane, const float* vp, const float* __restrict__ sx,
+                           const Frame& f, int32_t tx, int32_t ty) {
+  const int32_t i0 = tx * f.TW, i1 = min(i0 + f.TW, f.W) - 1;
+  const int32_t j0 = ty * f.TH, j1 = min(j0 + f.TH, f.H) - 1;
+  return rect_may_cover(r0, r1, r2, sane, vp, sx, f, i0, i1, j0, j1);
 }
 
 // Triangles whose tile range spans more than this many tiles are binned by a
@@ -431,7 +477,7 @@ union SetupStage {
   SmallStage sm;
 };
 
-__global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
+__global__ __launch_bounds__(kSetupThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_setup(
     jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
     float4* __restrict__ rec, Rect* __restrict__ rects,
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
@@ -537,7 +583,8 @@ __global__ __launch_bounds__(kSetupThreads, 4) void k_setup(
     Rect px = r;
     // a NaN coefficient makes that edge value NaN at every pixel: never inside
     const bool nan_col = has_nan_column(r0) || has_nan_column(r1) || has_nan_column(r2);
-    if (keep && front && !nan_col) px = pixel_range(r0, r1, r2, sane, sc.viewport, f);
+    if (keep && front && !nan_col)
+      px = pixel_range(r0, r1, r2, sane, sc.viewport, samples + (int64_t)s * (f.W + f.H), f);
     const uint32_t flags = (keep ? kKeep : 0u) | (front ? kFront : 0u) | (sane ? kSane : 0u);
     const int64_t area = (px.tx0 > px.tx1)
         ? 0 : (int64_t)(px.tx1 - px.tx0 + 1) * (px.ty1 - px.ty0 + 1);
