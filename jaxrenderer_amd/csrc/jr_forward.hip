@@ -128,35 +128,15 @@ JR_DEV bool rect_may_cover(const float4& r0, const float4& r1, const float4& r2,
                            bool sane, const float* vp, const float* __restrict__ sx,
                            const Frame& f, int32_t i0, int32_t i1, int32_t j0, int32_t j1);
 
-// The viewport split into kFallbackSplit x kFallbackSplit rectangles, each
-// kept only where the exact rectangle test finds that a pixel of it may pass:
-// the range of a triangle whose region R has no usable bound (about one
+// The range of a triangle whose region R has no usable bound (about one
 // triangle in a hundred in a Brax-like frame: edge-on or crossing the eye
-// plane; half of them cover no pixel at all).
-constexpr int32_t kFallbackSplit = 1;
-
+// plane): the whole viewport, unless the exact rectangle test finds that no
+// pixel of it can pass (half of them cover no pixel at all).
 JR_DEV Rect fallback_range(const float4& r0, const float4& r1, const float4& r2, bool sane,
                            const float* vp, const float* __restrict__ sx, const Frame& f) {
-  Rect r = {1, 0, 1, 0};
-#pragma unroll
-  for (int32_t qi = 0; qi < kFallbackSplit; ++qi) {
-    const int32_t i0 = (f.W * qi) / kFallbackSplit, i1 = (f.W * (qi + 1)) / kFallbackSplit - 1;
-#pragma unroll
-    for (int32_t qj = 0; qj < kFallbackSplit; ++qj) {
-      const int32_t j0 = (f.H * qj) / kFallbackSplit, j1 = (f.H * (qj + 1)) / kFallbackSplit - 1;
-      if (i0 > i1 || j0 > j1 || !rect_may_cover(r0, r1, r2, sane, vp, sx, f, i0, i1, j0, j1))
-        continue;
-      if (r.tx0 > r.tx1) {
-        r = {i0, i1, j0, j1};
-      } else {
-        r.tx0 = min(r.tx0, i0);
-        r.tx1 = max(r.tx1, i1);
-        r.ty0 = min(r.ty0, j0);
-        r.ty1 = max(r.ty1, j1);
-      }
-    }
-  }
-  return r;
+  if (!rect_may_cover(r0, r1, r2, sane, vp, sx, f, 0, f.W - 1, 0, f.H - 1))
+    return Rect{1, 0, 1, 0};
+  return Rect{0, f.W - 1, 0, f.H - 1};
 }
 
 // sx: the scene's sample table
