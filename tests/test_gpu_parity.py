@@ -68,6 +68,34 @@ def test_specular_exponents_fast_and_exact_power(shader):
     assert (got["main"]["prim_id"].cpu().numpy() >= 0).sum() > 500
 
 
+@pytest.mark.parametrize("tile", [(8, 8), (16, 16)], ids=str)
+@pytest.mark.parametrize("seed", [0, 1])
+def test_unbounded_and_edge_on_triangles(tile, seed, monkeypatch):
+    """Triangles whose pixel region has no bounded range: a tenth of the soup
+    reaches behind the eye, and a fifth lies in planes through the eye
+    (edge-on, projecting to a line).  Their range is the whole viewport only
+    where the exact rectangle test cannot reject it; ids and depth stay
+    bit-exact against the oracle, colours within 1e-5."""
+    soup = scenes._soup
+    eye = np.array((0.3, 0.2, 3.0), np.float32)
+
+    def wild(rng, n_tris, behind=0.1, degenerate=0.05):
+        pos = soup(rng, n_tris, behind=behind, degenerate=degenerate).reshape(n_tris, 3, 3)
+        # in front of the eye (the camera looks down -z from z = 3), in a
+        # plane through it
+        k = rng.random(n_tris) < 0.2
+        d = rng.normal(size=(2, n_tris, 3)).astype(np.float32)
+        d[..., 2] = -np.abs(d[..., 2]) - 0.5
+        ab = rng.uniform(0.5, 3.0, size=(n_tris, 3, 2)).astype(np.float32)
+        plane = eye + ab[..., :1] * d[0][:, None, :] + ab[..., 1:] * d[1][:, None, :]
+        pos[k] = plane[k]
+        return pos.reshape(-1, 3).astype(np.float32)
+
+    monkeypatch.setattr(scenes, "_soup", wild)
+    _check(lambda: scenes.soup_call(PhongReflectionShadowTextureShader, 40 + seed, 72, 56, 300),
+           f"unbounded/{seed}/{tile}", tile=tile)
+
+
 @pytest.mark.parametrize("tile", TILES, ids=str)
 @pytest.mark.parametrize("persistent", [False, True])
 def test_tiles_and_persistent(tile, persistent):
