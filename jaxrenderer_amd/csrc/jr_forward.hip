@@ -1057,6 +1057,9 @@ JR_DEV float shadow_depth_on_demand(const ShadowLookup& L, int32_t s, float x, f
 // source: the shadow pass's binning.
 struct LookupShadowDepth {
   const ShadowLookup* L;
+  // kept texels are read back by the backward: every one a pixel projects to
+  // is computed
+  JR_DEV bool required() const { return L->texels != nullptr; }
   JR_DEV float operator()(const jr_inputs&, int scene, float x, float y) const {
     return shadow_depth_on_demand(*L, scene, x, y);
   }

@@ -123,6 +123,8 @@ JR_DEV float shadow_get(const float* __restrict__ map, int32_t sw, int32_t sh,
 // from a finished map (in.shadow_maps).  A raster may hand the shadow shader
 // another source of the same depths.
 struct MapShadowDepth {
+  // a read whose result the colour does not depend on may be skipped
+  JR_DEV bool required() const { return false; }
   JR_DEV float operator()(const jr_inputs& in, int scene, float x, float y) const {
     const float* map = in.shadow_maps + (int64_t)scene * in.shadow_w * in.shadow_h;
     return shadow_get(map, in.shadow_w, in.shadow_h, x, y);
@@ -547,20 +549,29 @@ struct Varyings<JR_SHADER_PHONG_REFLECTION_SHADOW> {
     uv[1] = interp(b, uvv[1], uvv[3], uvv[5]);
 #pragma unroll
     for (int q = 0; q < 4; ++q) s4[q] = interp(b, scd[q], scd[4 + q], scd[8 + q]);
-    // shadow.camera.viewport @ shadow_coord, then normalise_homogeneous
-    float sv[4];
-    apply4(sc.shadow_viewport, s4[0], s4[1], s4[2], s4[3], sv);
-    const float sx = sv[0] / sv[3], sy = sv[1] / sv[3], sz = sv[2] / sv[3];
-    const float sd = shadow_depth(in, scene, sx, sy);
-    const bool lit = sz <= sd;
     float tex[3], diff, spec;
     reflection_terms(in, sc, n, uv, ti, __float_as_int(v[kShape]), __float_as_int(v[kShape + 1]), l,
                      tex, &diff, &spec);
+    float k[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) k[ch] = sc.diffuse[ch] * diff + sc.specular[ch] * spec;
+    // The shadow test only scales k: where all three are zero (a surface
+    // facing away from the light that reflects none of it towards the eye)
+    // shadow * k is a zero either way and the colour does not depend on the
+    // test, so it is skipped unless its depth must be computed anyway.
+    bool lit = true;
+    if (shadow_depth.required() || k[0] != 0.0f || k[1] != 0.0f || k[2] != 0.0f) {
+      // shadow.camera.viewport @ shadow_coord, then normalise_homogeneous
+      float sv[4];
+      apply4(sc.shadow_viewport, s4[0], s4[1], s4[2], s4[3], sv);
+      const float sx = sv[0] / sv[3], sy = sv[1] / sv[3], sz = sv[2] / sv[3];
+      const float sd = shadow_depth(in, scene, sx, sy);
+      lit = sz <= sd;
+    }
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
       const float shadow = lit ? 1.0f : 1.0f - sc.shadow_strength[ch];
-      const float k = sc.diffuse[ch] * diff + sc.specular[ch] * spec;
-      col[ch] = sc.ambient[ch] * tex[ch] + ((shadow * k) * tex[ch]) * sc.light_colour[ch];
+      col[ch] = sc.ambient[ch] * tex[ch] + ((shadow * k[ch]) * tex[ch]) * sc.light_colour[ch];
     }
     return front;
   }
