@@ -248,6 +248,11 @@ JR_DEV bool tile_may_cover(const float4& r0, const float4& r1, const float4& r2,
 // whole wavefront (one lane per tile) instead of by their own setup thread.
 constexpr int32_t kWideTiles = 16;
 
+// Offset in words from the wide-list length to k_bin's list length in the
+// workspace's misc words: 128 bytes apart, so their atomics do not share a
+// cache line.
+constexpr int32_t kBinListAt = 32;
+
 // Per-triangle binning word written by k_setup: kMaskNone (culled, small,
 // or no tile may hold a covered pixel: nothing to bin and no tile range
 // stored), kMaskWide (binned by k_wide), else bit q set for each counted tile
@@ -463,9 +468,11 @@ __global__ __launch_bounds__(kSetupThreads) __attribute__((amdgpu_waves_per_eu(8
     int32_t* __restrict__ counts, uint64_t* __restrict__ keys,
     int2* __restrict__ wide, int32_t* __restrict__ wide_count,
     const float* __restrict__ samples, const uint32_t* __restrict__ ks,
-    uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene) {
+    uint32_t* __restrict__ masks, const int32_t* __restrict__ wave_scene,
+    int2* __restrict__ blist) {
   __shared__ SetupStage stage;
   __shared__ int32_t wave_live[kSetupThreads / 64];
+  __shared__ int32_t bl_n[kSetupThreads / 64 + 1];  // list entries per wavefront, then the base
   // wave_owner scratch, 64 per wavefront, in bytes: with 4-byte entries the
   // workgroup's LDS would pass 20 KiB, an eighth of the CU's 160 KiB, and 7
   // workgroups would fit on a CU instead of 8
@@ -649,7 +656,29 @@ __global__ __launch_bounds__(kSetupThreads) __attribute__((amdgpu_waves_per_eu(8
     }
   }
   __syncthreads();
-  if (n_tile > 0) masks[t] = sm.u.t.tmask[threadIdx.x];
+  // the counted triangles with a tile to bin go on k_bin's list as
+  // (triangle, scene), the workgroup's entries together: one atomic per
+  // workgroup on the list length (its own cache line, kBinListAt words past
+  // wide_count)
+  {
+    const uint32_t tm = n_tile > 0 ? sm.u.t.tmask[threadIdx.x] : 0u;
+    if (n_tile > 0) masks[t] = tm;
+    const uint64_t bb = __ballot(tm != 0u);
+    if (lane == 0) bl_n[threadIdx.x >> 6] = (int32_t)__popcll(bb);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int32_t total = 0;
+#pragma unroll
+      for (int w = 0; w < kSetupThreads / 64; ++w) total += bl_n[w];
+      bl_n[kSetupThreads / 64] = total > 0 ? atomicAdd(wide_count + kBinListAt, total) : 0;
+    }
+    __syncthreads();
+    if (tm != 0u) {
+      int32_t at = bl_n[kSetupThreads / 64] + (int32_t)__popcll(bb & ((1ull << lane) - 1ull));
+      for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) at += bl_n[w];
+      blist[at] = make_int2(t, s);
+    }
+  }
   // The sample table of the workgroup's small triangles' scene goes to LDS
   // when they all share one scene and it fits: the loop below then reads no
   // global memory, and its key atomics never hold up a load (a wavefront's
@@ -860,54 +889,57 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(
 // each lane claiming its tiles one after another.  All staging is
 // wavefront-local.
 __global__ __launch_bounds__(256) void k_bin(
-    jr_inputs in, const jr_scene* __restrict__ scenes, Frame f,
-    const Rect* __restrict__ rects,
+    Frame f, const Rect* __restrict__ rects,
     const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
     int32_t* __restrict__ bins, const uint32_t* __restrict__ masks,
-    const int32_t* __restrict__ wave_scene) {
+    const int2* __restrict__ blist, const int32_t* __restrict__ blist_n) {
   __shared__ int4 own[256];  // tile range (tx0, ty0, height in tiles) and s * T
   __shared__ int32_t own_t[256];
   __shared__ uint32_t own_m[256];
   __shared__ int32_t own_off[256];  // exclusive prefix sum of the bit counts
   __shared__ int8_t mark[256];      // wave_owner scratch, 64 per wavefront
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (__builtin_amdgcn_readfirstlane(t & ~63) >= in.n_tris) return;
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
-  const int32_t s = scene_of_wave(scenes, f.n_scenes, wave_scene, t, in.n_tris);
-  int32_t n_b = 0;
-  if (t < in.n_tris) {
-    const uint32_t mask = masks[t];  // the tiles k_setup counted
-    if (mask != kMaskNone && mask != kMaskWide) {  // (k_wide bins the wide ones)
-      const Rect r = rects[t];
-      own[threadIdx.x] = make_int4(r.tx0, r.ty0, r.ty1 - r.ty0 + 1, s * f.T);
-      own_t[threadIdx.x] = t;
+  const int32_t n = *blist_n;
+  // the wavefronts stride over k_setup's list, 64 entries at a time
+  const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int32_t n_waves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+  for (int32_t first = wave * 64; first < n; first += n_waves * 64) {
+    const int32_t i = first + lane;
+    int32_t n_b = 0;
+    if (i < n) {
+      const int2 ts = blist[i];  // (triangle, scene)
+      const uint32_t mask = masks[ts.x];  // the tiles k_setup counted
+      const Rect r = rects[ts.x];
+      own[threadIdx.x] = make_int4(r.tx0, r.ty0, r.ty1 - r.ty0 + 1, ts.y * f.T);
+      own_t[threadIdx.x] = ts.x;
       own_m[threadIdx.x] = mask;
       n_b = __popc(mask);
     }
-  }
-  int32_t incl = n_b;
+    int32_t incl = n_b;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int32_t v = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += v;
-  }
-  const int32_t total = __shfl(incl, 63, 64);
-  own_off[threadIdx.x] = incl - n_b;
-  tile_sync<true>();
-  for (int32_t base = 0; base < total; base += 64) {
-    const int32_t item = base + lane;
-    const int o = wbase + wave_owner(incl - n_b, n_b, base, mark + wbase, lane);
-    if (item >= total) continue;
-    uint32_t m = own_m[o];
-    for (int32_t k = item - own_off[o]; k > 0; --k) m &= m - 1u;  // k-th set bit
-    const int q = __ffs(m) - 1;
-    const int4 box = own[o];
-    const int32_t dq = div_small(q, box.z);
-    const int32_t tx = box.x + dq, ty = box.y + (q - dq * box.z);
-    const int64_t gt = (int64_t)box.w + tx * f.tiles_y + ty;
-    const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
-    if (pos < f.capacity) bins[pos] = own_t[o];
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    const int32_t total = __shfl(incl, 63, 64);
+    own_off[threadIdx.x] = incl - n_b;
+    tile_sync<true>();
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t item = base + lane;
+      const int o = wbase + wave_owner(incl - n_b, n_b, base, mark + wbase, lane);
+      if (item >= total) continue;
+      uint32_t m = own_m[o];
+      for (int32_t k = item - own_off[o]; k > 0; --k) m &= m - 1u;  // k-th set bit
+      const int q = __ffs(m) - 1;
+      const int4 box = own[o];
+      const int32_t dq = div_small(q, box.z);
+      const int32_t tx = box.x + dq, ty = box.y + (q - dq * box.z);
+      const int64_t gt = (int64_t)box.w + tx * f.tiles_y + ty;
+      const int64_t pos = (int64_t)offsets[gt] + atomicAdd(cursor + gt, 1);
+      if (pos < f.capacity) bins[pos] = own_t[o];
+    }
+    tile_sync<true>();  // the entries' staging is rewritten by the next step
   }
 }
 
@@ -1566,7 +1598,7 @@ int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct WsLayout {
   int64_t rec, rects, counts, cursor, offsets, block_sums, misc, keys, kstate, wide, samples,
-      masks, wave_scene, wbits, bins, total;
+      masks, wave_scene, wbits, bins, blist, total;
   int64_t n_tiles, nblk, n_pix;
 };
 
@@ -1620,7 +1652,8 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.rects = o;      o += align_up(16 * (int64_t)std::max(n_tris, 1));
   L.counts = o;     o += align_up(4 * L.n_tiles);
   L.cursor = o;     o += align_up(4 * L.n_tiles);
-  L.misc = o;       o += align_up(64);   // [0] total pairs, [1] work counter, [2] wide count
+  L.misc = o;       o += align_up(256);  // [0] total pairs, [1] work counter, [2] wide count,
+                                         // [2 + kBinListAt] k_bin's list length
   L.offsets = o;    o += align_up(4 * (L.n_tiles + 1));
   L.block_sums = o; o += align_up(4 * std::max<int64_t>(L.nblk, 1));
   L.n_pix = (int64_t)fr->width * fr->height * fr->n_scenes;
@@ -1631,6 +1664,7 @@ WsLayout ws_layout(const jr_frame* fr, int32_t n_tris) {
   L.wave_scene = o; o += align_up(4 * (((int64_t)std::max(n_tris, 1) + 63) / 64));
   L.wbits = o;      o += align_up(4 * (int64_t)kWideBitWords);
   L.bins = o;       o += align_up(4 * std::max<int64_t>(fr->bin_capacity, 1));
+  L.blist = o;      o += align_up(8 * (int64_t)std::max(n_tris, 1));
   L.total = o;
   return L;
 }
@@ -1851,6 +1885,7 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
   int32_t* wave_scene = reinterpret_cast<int32_t*>(ws + L.wave_scene);
   int32_t* wide_count = misc + 2;
   uint32_t* wbits = reinterpret_cast<uint32_t*>(ws + L.wbits);
+  int2* blist = reinterpret_cast<int2*>(ws + L.blist);
   const unsigned wide_grid = 8192;  // 32768 wavefronts stride over the wide list
 
   // a binning-only call is timed too (its raster stage then takes no time)
@@ -1876,7 +1911,7 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
     hipLaunchKernelGGL(k_setup, dim3((n_tris + kSetupThreads - 1) / kSetupThreads),
                        dim3(kSetupThreads), 0, st, *in,
                        scenes, f, rec, rects, counts, keys, wide, wide_count, samples, kstate,
-                       masks, wave_scene);
+                       masks, wave_scene, blist);
   timer.mark(2);
   if (n_tris > 0)
     hipLaunchKernelGGL(k_wide<false>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
@@ -1890,8 +1925,9 @@ int32_t forward_body(const jr_frame* fr, const jr_inputs* in, const jr_scene* sc
                      dim3(kScanThreads), 0, st, offsets, block_sums, L.n_tiles);
   timer.mark(4);
   if (n_tris > 0) {
-    hipLaunchKernelGGL(k_bin, dim3((n_tris + 255) / 256), dim3(256), 0, st, *in,
-                       scenes, f, rects, offsets, cursor, bins, masks, wave_scene);
+    hipLaunchKernelGGL(k_bin, dim3((unsigned)std::min((n_tris + 255) / 256, 2048)), dim3(256), 0,
+                       st, f, rects, offsets, cursor, bins, masks, blist,
+                       wide_count + kBinListAt);
     hipLaunchKernelGGL(k_wide<true>, dim3(wide_grid), dim3(256), 0, st, scenes, f, rec,
                        rects, wide, wide_count, counts, offsets, cursor, bins, samples, wbits);
   }
