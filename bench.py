@@ -163,8 +163,9 @@ def main() -> None:
                          "kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build renders on one GPU; --gpus must be 1 "
-                                   "(scenes shard across GPUs with no exchange, see DESIGN.md)"}))
+        print(json.dumps({"error": "bench.py renders on one GPU; --gpus must be 1 (a batch "
+                                   "shards by scene across GPUs with no exchange: one process "
+                                   "per GPU, each rendering its own scenes)"}))
         sys.exit(2)
 
     import numpy as np
