@@ -229,6 +229,41 @@ def test_depth_pass_keys_carry_over_calls():
             assert_parity(got[tag], ref[tag], label=f"{label}/{tag}")
 
 
+def test_depth_pass_on_a_garbage_workspace():
+    """A workspace whose bytes are random (memory that held something else)
+    does not describe its tagged keys: the depth pass refills them, so two
+    calls on it, the second after the keys were overwritten with other random
+    bytes again, both match the oracle bit for bit."""
+    from jaxrenderer_amd import _raster, batch, workloads
+
+    scs, sm = workloads.config3(n_scenes=2, seed=51)
+    call = [c for c in batch.plan_batch(scs, 84, 84, sm) if c.tag == "shadow"][0]
+    ref = run_oracle_plan([call], prim_id=False, bary=False)[call.tag]
+    dev = torch.device("cuda", 0)
+    B, W, H = call.n_scenes, call.width, call.height
+    need = _raster.workspace_size(width=W, height=H, n_scenes=B, n_tris=call.n_tris,
+                                  capacity=_raster.default_capacity(call.n_tris,
+                                                                    _raster.n_tiles(W, H, B)))
+    gen = torch.Generator(device=dev).manual_seed(5)
+    ws = torch.randint(0, 256, (need,), dtype=torch.uint8, device=dev, generator=gen)
+    inputs = _plan.device_inputs(call, dev)
+    scenes_dev = _raster.scenes_to_device(call.scenes, dev)
+    for k in range(2):
+        if k == 1:
+            ws.copy_(torch.randint(0, 256, (need,), dtype=torch.uint8, device=dev,
+                                   generator=gen))
+        zbuffer = torch.empty((B, W, H), dtype=torch.float32, device=dev)
+        _raster.forward(shader=call.shader, width=W, height=H, scenes=scenes_dev, n_scenes=B,
+                        inputs=inputs, n_slots=call.n_slots, n_tris=call.n_tris,
+                        zbuffer=zbuffer, canvas=None, prim_id=None, bary=None, display=None,
+                        shadow_shape=call.shadow_shape, init_buffers=call.init_buffers,
+                        zbuffer_default=call.zbuffer_default,
+                        colour_default=call.colour_default, apply_offset=call.apply_offset,
+                        depth_offset=call.depth_offset, workspace=ws)
+        torch.cuda.synchronize()
+        assert_parity({"zbuffer": zbuffer}, ref, label=f"garbage workspace, call {k}")
+
+
 @pytest.mark.parametrize("shader", [DepthShader, PhongReflectionShadowTextureShader],
                          ids=lambda s: s.__name__)
 def test_ragged_and_empty_scenes(shader):
