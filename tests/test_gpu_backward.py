@@ -41,8 +41,8 @@ def _shadow_map(shadow_call):
     return _plan.run(shadow_call)["zbuffer"][0].cpu().numpy()
 
 
-def _setup(shader, seed):
-    cam, faces, extra, shadow_call = scenes.soup_inputs(shader, seed, W, H)
+def _setup(shader, seed, w=W, h=H):
+    cam, faces, extra, shadow_call = scenes.soup_inputs(shader, seed, w, h)
     if hasattr(extra, "specular_map"):
         # exponents below 1 make pow's derivative at 0 infinite and jax.grad's
         # (and this backward's) gradient NaN wherever the specular term is 0
@@ -115,8 +115,22 @@ def _ref_params(L, extra, shadow_cam):
 @pytest.mark.parametrize("shader", SHADERS, ids=lambda s: s.__name__)
 @pytest.mark.parametrize("seed", [0, 1])
 def test_backward_matches_float64_autograd(shader, seed):
+    _check_against_float64(shader, seed, W, H)
+
+
+# The backward's wavefronts take the full 16 x 16 tiles first, then the 2x2
+# pixel blocks below them and right of them packed together: odd frames have
+# both kinds of leftover blocks and blocks cut by the frame's edge.
+@pytest.mark.parametrize("shader", [GouraudShader, PhongReflectionShadowTextureShader],
+                         ids=lambda s: s.__name__)
+@pytest.mark.parametrize("size", [(37, 29), (51, 39)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_backward_on_odd_frame_sizes(shader, size):
+    _check_against_float64(shader, 0, *size)
+
+
+def _check_against_float64(shader, seed, W, H):
     dev = torch.device("cuda")
-    cam, faces, extra = _setup(shader, seed)
+    cam, faces, extra = _setup(shader, seed, W, H)
     call = build_call(cam, shader, faces, extra, W, H, init_buffers=True)
     pid = _plan.run(call, prim_id=True)["prim_id"][0].cpu().numpy()
     assert (pid >= 0).sum() > 50
