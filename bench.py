@@ -173,13 +173,6 @@ def main() -> None:
 
     from jaxrenderer_amd import _hip, batch, workloads
 
-    if args.workload == "c4":
-        print(json.dumps(run_views(args)), flush=True)
-        return
-    if args.workload == "c5":
-        print(json.dumps(run_large(args)), flush=True)
-        return
-
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -187,6 +180,13 @@ def main() -> None:
         torch.cuda.synchronize()
         log("calibration copy: 268435456 bytes read, 268435456 bytes written")
         del src, dst
+
+    if args.workload == "c4":
+        print(json.dumps(run_views(args)), flush=True)
+        return
+    if args.workload == "c5":
+        print(json.dumps(run_large(args)), flush=True)
+        return
 
     W = H = 84
     B = args.scenes
